@@ -1,0 +1,354 @@
+"""ctypes binding of the CPU restatement (oracle/libkzoracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libkzoracle.so"
+
+
+def build(force=False):
+    """Compile the restatement with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", str(HERE)], check=True)
+
+
+def _pkg_types():
+    import importlib.util
+    import sys
+    name = "koral_zig_amd"
+    if name not in sys.modules:
+        root = HERE.parent / "koral-zig_amd"
+        spec = importlib.util.spec_from_file_location(name, root / "__init__.py",
+                                                      submodule_search_locations=[str(root)])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[name] = mod
+        spec.loader.exec_module(mod)
+    from koral_zig_amd import abi_types
+    return abi_types
+
+
+T = _pkg_types()
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.kzo_sim_create.restype = vp
+        L.kzo_sim_create.argtypes = [C.POINTER(T.kz_grid), C.POINTER(T.kz_physics)] + [_D] * 9 + [_D, _I]
+        for n in ("kzo_sim_destroy", "kzo_sim_p2u_domain", "kzo_sim_init_timestep_guess", "kzo_flux_ct"):
+            getattr(L, n).argtypes = [vp]
+            getattr(L, n).restype = None
+        L.kzo_sim_set_state.argtypes = [vp, _D, _D]
+        L.kzo_sim_set_state.restype = None
+        L.kzo_sim_get_state.argtypes = [vp, _D, _D, _I]
+        L.kzo_sim_get_state.restype = None
+        L.kzo_sim_get_scal.argtypes = [vp, _D]
+        L.kzo_sim_get_scal.restype = None
+        L.kzo_sim_get_flb.argtypes = [vp, C.c_int, _D]
+        L.kzo_sim_get_flb.restype = None
+        L.kzo_sim_counters.argtypes = [vp, C.POINTER(T.kz_counters)]
+        L.kzo_sim_counters.restype = None
+        L.kzo_sim_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.kzo_sim_set_time.restype = None
+        L.kzo_sim_set_bc.argtypes = [vp, C.c_int]
+        L.kzo_sim_calc_u2p.argtypes = [vp]
+        L.kzo_sim_op_explicit.argtypes = [vp, C.c_double]
+        L.kzo_sim_op_implicit.argtypes = [vp, C.c_double]
+        L.kzo_sim_do_correct.argtypes = [vp]
+        L.kzo_sim_step.argtypes = [vp, _D]
+        L.kzo_sim_postinit.argtypes = [vp, C.c_double]
+        L.kzo_sim_postinit.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _geoms(g):
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    assert g.ndim == 2 and g.shape[1] == T.GEOM_DOUBLES
+    return g
+
+
+def p2u(pp, geom, gam):
+    pp = np.ascontiguousarray(pp, np.float64)
+    g = _geoms(geom)
+    uu = np.zeros_like(pp)
+    lib().kzo_p2u_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(g), C.c_double(gam), T.ptr(uu))
+    return uu
+
+
+def u2p_solver_w(uu, guess, geom, etype, gam):
+    uu = np.ascontiguousarray(uu, np.float64)
+    pp = np.array(guess, np.float64, copy=True, order="C")
+    ret = np.zeros(len(uu), np.int32)
+    lib().kzo_u2p_solver_w_batch(C.c_int64(len(uu)), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)),
+                                 C.c_int32(etype), C.c_double(gam), T.ptr(ret, C.c_int32))
+    return ret, pp
+
+
+def u2p_mhd(uu, guess, geom, gam, floors):
+    uu = np.ascontiguousarray(uu, np.float64)
+    pp = np.array(guess, np.float64, copy=True, order="C")
+    res = np.zeros(len(uu), T.U2P_RESULT_DTYPE)
+    lib().kzo_u2p_mhd_batch(C.c_int64(len(uu)), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam),
+                            C.byref(floors), res.ctypes.data_as(C.c_void_p))
+    return res, pp
+
+
+def check_floors_mhd(pp, geom, gam, floors):
+    pp = np.array(pp, np.float64, copy=True, order="C")
+    ret = np.zeros(len(pp), np.int32)
+    ng = C.c_uint64(0)
+    lib().kzo_check_floors_mhd_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam),
+                                     C.byref(floors), T.ptr(ret, C.c_int32), C.byref(ng))
+    return ret, pp, ng.value
+
+
+def u2p_rad(uu, guess, geom, rp):
+    uu = np.ascontiguousarray(uu, np.float64)
+    pp = np.array(guess, np.float64, copy=True, order="C")
+    res = np.zeros(len(uu), T.U2PRAD_RESULT_DTYPE)
+    lib().kzo_u2p_rad_batch(C.c_int64(len(uu)), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), C.byref(rp),
+                            res.ctypes.data_as(C.c_void_p))
+    return res, pp
+
+
+def check_floors_rad(pp, geom, rp):
+    pp = np.array(pp, np.float64, copy=True, order="C")
+    ret = np.zeros(len(pp), np.int32)
+    lib().kzo_check_floors_rad_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.byref(rp),
+                                     T.ptr(ret, C.c_int32))
+    return ret, pp
+
+
+def solve_implicit(pp, geom, dt, gam, rp, op, ip):
+    pp = np.array(pp, np.float64, copy=True, order="C")
+    uu = np.zeros_like(pp)
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (len(pp),)))
+    res = np.zeros(len(pp), T.IMPLICIT_RESULT_DTYPE)
+    lib().kzo_solve_implicit_lab_batch(C.c_int64(len(pp)), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(dt),
+                                       C.c_double(gam), C.byref(rp), C.byref(op), C.byref(ip),
+                                       res.ctypes.data_as(C.c_void_p))
+    return res, pp, uu
+
+
+def reconstruct(order, theta, u5, dx5):
+    n = len(u5)
+    order = np.ascontiguousarray(np.broadcast_to(np.asarray(order, np.int32), (n,)))
+    theta = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float64), (n,)))
+    l = np.zeros(n)
+    r = np.zeros(n)
+    lib().kzo_reconstruct_batch(C.c_int64(n), T.ptr(order, C.c_int32), T.ptr(theta),
+                                T.ptr(np.ascontiguousarray(u5, np.float64)),
+                                T.ptr(np.ascontiguousarray(dx5, np.float64)), T.ptr(l), T.ptr(r))
+    return l, r
+
+
+def laxf(fl, fr, ul, ur, ag):
+    a = [np.ascontiguousarray(x, np.float64) for x in (fl, fr, ul, ur, ag)]
+    out = np.zeros_like(a[0])
+    lib().kzo_laxf_batch(C.c_int64(len(out)), *[T.ptr(x) for x in a], T.ptr(out))
+    return out
+
+
+def flux_prime(pp, idim, geom, gam):
+    pp = np.ascontiguousarray(pp, np.float64)
+    n = len(pp)
+    idim = np.ascontiguousarray(np.broadcast_to(np.asarray(idim, np.int32), (n,)))
+    ff = np.zeros_like(pp)
+    err = np.zeros(n, np.int32)
+    lib().kzo_flux_prime_batch(C.c_int64(n), T.ptr(pp), T.ptr(idim, C.c_int32), T.ptr(_geoms(geom)),
+                               C.c_double(gam), T.ptr(ff), T.ptr(err, C.c_int32))
+    return ff, err
+
+
+def gas_wavespeeds(pp, geom, gam, active=(1, 1, 0)):
+    pp = np.ascontiguousarray(pp, np.float64)
+    a = np.zeros((len(pp), 6))
+    act = np.array(active, np.int32)
+    lib().kzo_gas_wavespeeds_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam),
+                                   T.ptr(act, C.c_int32), T.ptr(a))
+    return a
+
+
+def rad_wavespeeds(pp, geom, tau3, active=(1, 1, 0)):
+    pp = np.ascontiguousarray(pp, np.float64)
+    a = np.zeros((len(pp), 12))
+    act = np.array(active, np.int32)
+    lib().kzo_rad_wavespeeds_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)),
+                                   T.ptr(np.ascontiguousarray(tau3, np.float64)), T.ptr(act, C.c_int32), T.ptr(a))
+    return a
+
+
+def rij(pp, geom):
+    pp = np.ascontiguousarray(pp, np.float64)
+    out = np.zeros((len(pp), 21))
+    lib().kzo_rij_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(out))
+    return out
+
+
+def tij(pp, geom, gam):
+    pp = np.ascontiguousarray(pp, np.float64)
+    out = np.zeros((len(pp), 33))
+    lib().kzo_tij_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam), T.ptr(out))
+    return out
+
+
+def radstate(pp, geom, gam, op, full=1):
+    pp = np.ascontiguousarray(pp, np.float64)
+    out = np.zeros((len(pp), 14))
+    lib().kzo_radstate_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam), C.byref(op),
+                             C.c_int32(full), T.ptr(out))
+    return out
+
+
+def opac_entry(pp, geom, gam, op):
+    pp = np.ascontiguousarray(pp, np.float64)
+    out = np.zeros((len(pp), 4))
+    lib().kzo_opac_entry_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam), C.byref(op),
+                               T.ptr(out))
+    return out
+
+
+def gi(pp, geom, gam, op):
+    pp = np.ascontiguousarray(pp, np.float64)
+    out = np.zeros((len(pp), 8))
+    slim = np.zeros((len(pp), 8))
+    lib().kzo_gi_batch(C.c_int64(len(pp)), T.ptr(pp), T.ptr(_geoms(geom)), C.c_double(gam), C.byref(op),
+                       T.ptr(out), T.ptr(slim))
+    return out, slim
+
+
+def invert4(a16):
+    a16 = np.ascontiguousarray(a16, np.float64).reshape(-1, 16)
+    out = np.zeros_like(a16)
+    lib().kzo_invert4_batch(C.c_int64(len(a16)), T.ptr(a16), T.ptr(out))
+    return out
+
+
+MATH_FN = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "log1p": 4, "sqrt": 5, "div": 6}
+
+
+def math_fn(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float64)
+    yy = None if y is None else np.ascontiguousarray(np.broadcast_to(np.asarray(y, np.float64), x.shape))
+    out = np.zeros_like(x)
+    lib().kzo_math_batch(C.c_int64(x.size), C.c_int32(MATH_FN[name]), T.ptr(x), T.ptr(yy), T.ptr(out))
+    return out
+
+
+class Sim:
+    """The restated Sim(config.puffy) over caller-built metric tables.
+
+    `tables` is the dict from koral_zig_amd.host.metric_tables(); arrays are
+    referenced, not copied, so the dict must outlive the Sim.
+    """
+
+    def __init__(self, grid, phys, tables):
+        self.grid = grid
+        self.phys = phys
+        self.tables = tables
+        err = C.c_int32(0)
+        t = tables
+        self.h = lib().kzo_sim_create(C.byref(grid), C.byref(phys), T.ptr(t["g"]), T.ptr(t["gcon"]),
+                                      T.ptr(t["kris"]), T.ptr(t["gb0"]), T.ptr(t["gconb0"]), T.ptr(t["gb1"]),
+                                      T.ptr(t["gconb1"]), T.ptr(t["gb2"]), T.ptr(t["gconb2"]), T.ptr(t["bcx"]),
+                                      C.byref(err))
+        if not self.h:
+            raise RuntimeError(f"kzo_sim_create failed: {T.ERRORS.get(err.value, err.value)}")
+        (sz, sy, sx), _ = T.padded_shape(grid)
+        self.shape = (sz, sy, sx)
+        self.ncell = sz * sy * sx
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kzo_sim_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"oracle {what}: {T.ERRORS.get(rc, rc)}")
+
+    def set_state(self, p=None, u=None):
+        p = None if p is None else np.ascontiguousarray(p, np.float64)
+        u = None if u is None else np.ascontiguousarray(u, np.float64)
+        lib().kzo_sim_set_state(self.h, T.ptr(p), T.ptr(u))
+
+    def get_state(self):
+        p = np.zeros(self.shape + (T.NV,))
+        u = np.zeros(self.shape + (T.NV,))
+        f = np.zeros(self.shape + (T.NFLAGS,), np.int32)
+        lib().kzo_sim_get_state(self.h, T.ptr(p), T.ptr(u), T.ptr(f, C.c_int32))
+        return p, u, f
+
+    def get_scal(self):
+        s = np.zeros(self.shape + (20,))
+        lib().kzo_sim_get_scal(self.h, T.ptr(s))
+        return s
+
+    def counters(self):
+        c = T.kz_counters()
+        lib().kzo_sim_counters(self.h, C.byref(c))
+        return {n: getattr(c, n) for n, _ in T.kz_counters._fields_}
+
+    def set_time(self, t, tstepdenmax, tstepdenmin):
+        lib().kzo_sim_set_time(self.h, t, tstepdenmax, tstepdenmin)
+
+    def p2u_domain(self):
+        lib().kzo_sim_p2u_domain(self.h)
+
+    def set_bc(self, ifinit=False):
+        self._chk(lib().kzo_sim_set_bc(self.h, int(ifinit)), "set_bc")
+
+    def calc_u2p(self):
+        self._chk(lib().kzo_sim_calc_u2p(self.h), "calc_u2p")
+
+    def op_explicit(self, dt):
+        self._chk(lib().kzo_sim_op_explicit(self.h, dt), "op_explicit")
+
+    def op_implicit(self, dt):
+        self._chk(lib().kzo_sim_op_implicit(self.h, dt), "op_implicit")
+
+    def do_correct(self):
+        self._chk(lib().kzo_sim_do_correct(self.h), "do_correct")
+
+    def init_timestep_guess(self):
+        lib().kzo_sim_init_timestep_guess(self.h)
+
+    def postinit(self, maxbeta):
+        return lib().kzo_sim_postinit(self.h, maxbeta)
+
+    def step(self, dt=None):
+        if dt is None:
+            self._chk(lib().kzo_sim_step(self.h, None), "step")
+        else:
+            d = C.c_double(dt)
+            self._chk(lib().kzo_sim_step(self.h, C.pointer(d)), "step")
+
+
+def true_residual(pp, pp00, dt, geom, gam, op, wp, we, wf):
+    """implicit_golden_tests.zig:25-40 trueResidual over n records."""
+    n = len(pp)
+    pp = np.ascontiguousarray(pp, np.float64)
+    pp00 = np.ascontiguousarray(pp00, np.float64)
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+    wp, we, wf = [np.ascontiguousarray(np.broadcast_to(np.asarray(x, np.int32), (n,))) for x in (wp, we, wf)]
+    err = np.zeros(n)
+    lib().kzo_true_residual_batch(C.c_int64(n), T.ptr(pp), T.ptr(pp00), T.ptr(dt), T.ptr(_geoms(geom)),
+                                  C.c_double(gam), C.byref(op), T.ptr(wp, C.c_int32), T.ptr(we, C.c_int32),
+                                  T.ptr(wf, C.c_int32), T.ptr(err))
+    return err

@@ -1,0 +1,934 @@
+/* kzo_sim.c — oracle restatement of koral/sim.zig's Sim(config.puffy) passes
+ * and RK2IMEX step, sim/ct.zig flux-CT, sim/bc.zig + the PUFFY Bc.calc
+ * (koral/problems/puffy/puffy.zig:1080-1194), sim/polaraxis.zig.
+ * TEST INFRASTRUCTURE (see kzo.h).
+ *
+ * Storage mirrors the reference exactly: AoS fields iv-fastest over the
+ * padded grid (koral/field.zig), FaceStore (+1 slice in its own dim,
+ * sim/storage.zig:62-135), MetricCache blocks (metric/precompute.zig:121-526)
+ * passed in by the caller.  Radiative viscosity and the mimic dynamo are not
+ * part of this path (their Options must be off); 2D (nz == 1) only.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "kzo.h"
+
+#define NSCAL 20
+enum {
+  S_AHDXL, S_AHDXR, S_AHDX, S_ARADXL, S_ARADXR, S_ARADX,
+  S_AHDYL, S_AHDYR, S_AHDY, S_ARADYL, S_ARADYR, S_ARADY,
+  S_AHDZL, S_AHDZR, S_AHDZ, S_ARADZL, S_ARADZR, S_ARADZ,
+  S_TSTEPDEN, S_CELL_DT
+};
+static const int ahd_l[3] = {S_AHDXL, S_AHDYL, S_AHDZL};
+static const int ahd_r[3] = {S_AHDXR, S_AHDYR, S_AHDZR};
+static const int ahd_m[3] = {S_AHDX, S_AHDY, S_AHDZ};
+static const int arad_l[3] = {S_ARADXL, S_ARADYL, S_ARADZL};
+static const int arad_r[3] = {S_ARADXR, S_ARADYR, S_ARADZR};
+static const int arad_m[3] = {S_ARADX, S_ARADY, S_ARADZ};
+
+#define BCX_REC 66 /* xhi ghost record: gBL[16] GBL[16] rghost rbound J12[16] J21[16] */
+
+typedef struct kzo_sim {
+  kz_grid grid;
+  int ngx, ngy, ngz, sx, sy, sz;
+  long ncell;
+  double dx, dy, dz;
+  kz_physics ph;
+  const double *g, *gcon, *kris;
+  const double *gb[3], *gconb[3];
+  const double *bcx;
+  double *p, *u, *ut0, *ut1, *ut2, *dut1, *dut2, *drt1, *drt2, *uforget, *u_bak, *p_bak;
+  double *pb_l[3], *pb_r[3], *fl_l[3], *fl_r[3], *flb[3];
+  long nface[3];
+  int fsx[3], fsy[3], fsz[3];
+  double *scal;
+  int32_t *flags;
+  double *emf[3];
+  double t, dt, own_dt, tstepdenmax, tstepdenmin, min_dx, min_dy, min_dz;
+  uint64_t nstep, n_fail, n_iters, n_solves, n_guard;
+} kzo_sim;
+
+/* ---- grid (koral/grid.zig:127-159) ---- */
+static double xl_(const kzo_sim *s, long i) { return s->grid.minx + (double)i * s->dx; }
+static double yl_(const kzo_sim *s, long i) { return s->grid.miny + (double)i * s->dy; }
+static double zl_(const kzo_sim *s, long i) { return s->grid.minz + (double)i * s->dz; }
+static double xc_(const kzo_sim *s, long i) { return 0.5 * (xl_(s, i) + xl_(s, i + 1)); }
+static double yc_(const kzo_sim *s, long i) { return 0.5 * (yl_(s, i) + yl_(s, i + 1)); }
+static double csize(const kzo_sim *s, long i, int dim) {
+  if (dim == 0) return xl_(s, i + 1) - xl_(s, i);
+  if (dim == 1) return yl_(s, i + 1) - yl_(s, i);
+  return zl_(s, i + 1) - zl_(s, i);
+}
+static long cidx(const kzo_sim *s, long ix, long iy, long iz) {
+  return (ix + s->ngx) + (iy + s->ngy) * (long)s->sx + (iz + s->ngz) * (long)s->sy * s->sx;
+}
+static double *cell(double *f, const kzo_sim *s, long ix, long iy, long iz) { return f + cidx(s, ix, iy, iz) * NV; }
+static long fidx(const kzo_sim *s, int dim, long ix, long iy, long iz) {
+  long jx = ix + s->ngx, jy = iy + s->ngy, jz = iz + s->ngz;
+  return ((jz * s->fsy[dim] + jy) * s->fsx[dim] + jx);
+}
+static double *face(double *f, const kzo_sim *s, int dim, long ix, long iy, long iz) {
+  return f + fidx(s, dim, ix, iy, iz) * NV;
+}
+static long mfidx(const kzo_sim *s, int dim, long ix, long iy, long iz) {
+  long jx = ix + s->ngx, jy = iy + s->ngy, jz = iz + s->ngz;
+  if (dim == 0) return jx + jy * (s->sx + 1) + jz * (long)s->sy * (s->sx + 1);
+  if (dim == 1) return jx + jy * s->sx + jz * (long)(s->sy + 1) * s->sx;
+  return jx + jy * s->sx + jz * (long)s->sy * s->sx;
+}
+static void geo_c(const kzo_sim *s, long ix, long iy, long iz, kzo_geo *g) {
+  long ci = cidx(s, ix, iy, iz);
+  kzo_geo_from_blocks(s->g + ci * 20, s->gcon + ci * 20, xc_(s, ix), g);
+}
+static void geo_f(const kzo_sim *s, long ix, long iy, long iz, int dim, kzo_geo *g) {
+  long fi = mfidx(s, dim, ix, iy, iz);
+  double x1 = dim == 0 ? xl_(s, ix) : xc_(s, ix);
+  kzo_geo_from_blocks(s->gb[dim] + fi * 20, s->gconb[dim] + fi * 20, x1, g);
+}
+static int32_t *flagp(kzo_sim *s, int f, long ix, long iy, long iz) { return s->flags + cidx(s, ix, iy, iz) * KZ_NFLAGS + f; }
+static double *scp(kzo_sim *s, int k, long ix, long iy, long iz) { return s->scal + cidx(s, ix, iy, iz) * NSCAL + k; }
+static long ndim(const kzo_sim *s, int d) { return d == 0 ? s->grid.nx : (d == 1 ? s->grid.ny : s->grid.nz); }
+
+/* ---- create / destroy ---- */
+kzo_sim *kzo_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, const double *gcon,
+                        const double *kris, const double *gb0, const double *gconb0, const double *gb1,
+                        const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
+  *err = 0;
+  if (gr->nz != 1 || gr->ny <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return 0; }
+  if (ph->radviscosity || ph->dynamo || ph->reduceorderatbh || ph->reduceorderafterfixup ||
+      ph->opac.synchrotron_bridge || ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return 0; }
+  if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return 0; }
+  kzo_sim *s = (kzo_sim *)calloc(1, sizeof(kzo_sim));
+  s->grid = *gr;
+  s->ph = *ph;
+  s->ngx = gr->nx > 1 ? gr->ng : 0;
+  s->ngy = gr->ny > 1 ? gr->ng : 0;
+  s->ngz = gr->nz > 1 ? gr->ng : 0;
+  s->sx = gr->nx + 2 * s->ngx;
+  s->sy = gr->ny + 2 * s->ngy;
+  s->sz = gr->nz + 2 * s->ngz;
+  s->ncell = (long)s->sx * s->sy * s->sz;
+  s->dx = (gr->maxx - gr->minx) / (double)gr->nx;
+  s->dy = (gr->maxy - gr->miny) / (double)gr->ny;
+  s->dz = (gr->maxz - gr->minz) / (double)gr->nz;
+  s->g = g; s->gcon = gcon; s->kris = kris;
+  s->gb[0] = gb0; s->gconb[0] = gconb0; s->gb[1] = gb1; s->gconb[1] = gconb1; s->gb[2] = gb2; s->gconb[2] = gconb2;
+  s->bcx = bcx;
+  size_t fb = (size_t)s->ncell * NV * sizeof(double);
+  double **fl[] = {&s->p, &s->u, &s->ut0, &s->ut1, &s->ut2, &s->dut1, &s->dut2, &s->drt1, &s->drt2, &s->uforget, &s->u_bak, &s->p_bak};
+  for (unsigned i = 0; i < sizeof(fl) / sizeof(fl[0]); ++i) *fl[i] = (double *)calloc(1, fb);
+  for (int d = 0; d < 3; ++d) {
+    s->fsx[d] = s->sx + (d == 0);
+    s->fsy[d] = s->sy + (d == 1);
+    s->fsz[d] = s->sz + (d == 2);
+    s->nface[d] = (long)s->fsx[d] * s->fsy[d] * s->fsz[d];
+    size_t b = (size_t)s->nface[d] * NV * sizeof(double);
+    s->pb_l[d] = (double *)calloc(1, b); s->pb_r[d] = (double *)calloc(1, b);
+    s->fl_l[d] = (double *)calloc(1, b); s->fl_r[d] = (double *)calloc(1, b);
+    s->flb[d] = (double *)calloc(1, b);
+  }
+  s->scal = (double *)calloc((size_t)s->ncell * NSCAL, sizeof(double));
+  s->flags = (int32_t *)calloc((size_t)s->ncell * KZ_NFLAGS, sizeof(int32_t));
+  long ncorn = (long)(gr->nx + 1) * (gr->ny + 1) * (gr->nz + 1);
+  for (int c = 0; c < 3; ++c) s->emf[c] = (double *)calloc(ncorn, sizeof(double));
+  s->tstepdenmin = 1.0e50;
+  /* set_grid min-size scan incl. its mdx quirk (sim.zig:447-466) */
+  double mdx = -1, mdy = -1, mdz = -1;
+  for (long ix = 0; ix < gr->nx; ++ix)
+    for (long iy = 0; iy < gr->ny; ++iy)
+      for (long iz = 0; iz < gr->nz; ++iz) {
+        double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, iz, 2);
+        if (dx < mdx || mdx < 0) mdx = dx;
+        if (dy < mdx || mdy < 0) mdy = dy;
+        if (dz < mdx || mdz < 0) mdz = dz;
+      }
+  s->min_dx = mdx; s->min_dy = mdy; s->min_dz = mdz;
+  return s;
+}
+
+void kzo_sim_destroy(kzo_sim *s) {
+  if (!s) return;
+  double *fl[] = {s->p, s->u, s->ut0, s->ut1, s->ut2, s->dut1, s->dut2, s->drt1, s->drt2, s->uforget, s->u_bak, s->p_bak};
+  for (unsigned i = 0; i < sizeof(fl) / sizeof(fl[0]); ++i) free(fl[i]);
+  for (int d = 0; d < 3; ++d) { free(s->pb_l[d]); free(s->pb_r[d]); free(s->fl_l[d]); free(s->fl_r[d]); free(s->flb[d]); }
+  free(s->scal); free(s->flags);
+  for (int c = 0; c < 3; ++c) free(s->emf[c]);
+  free(s);
+}
+
+void kzo_sim_set_state(kzo_sim *s, const double *p, const double *u) {
+  if (p) memcpy(s->p, p, (size_t)s->ncell * NV * sizeof(double));
+  if (u) memcpy(s->u, u, (size_t)s->ncell * NV * sizeof(double));
+}
+void kzo_sim_get_state(kzo_sim *s, double *p, double *u, int32_t *flags) {
+  if (p) memcpy(p, s->p, (size_t)s->ncell * NV * sizeof(double));
+  if (u) memcpy(u, s->u, (size_t)s->ncell * NV * sizeof(double));
+  if (flags) memcpy(flags, s->flags, (size_t)s->ncell * KZ_NFLAGS * sizeof(int32_t));
+}
+void kzo_sim_get_scal(kzo_sim *s, double *scal) { memcpy(scal, s->scal, (size_t)s->ncell * NSCAL * sizeof(double)); }
+void kzo_sim_get_flb(kzo_sim *s, int dim, double *out) { memcpy(out, s->flb[dim], (size_t)s->nface[dim] * NV * sizeof(double)); }
+void kzo_sim_counters(kzo_sim *s, kz_counters *c) {
+  c->t = s->t; c->dt = s->dt; c->own_dt = s->own_dt; c->tstepdenmax = s->tstepdenmax; c->tstepdenmin = s->tstepdenmin;
+  c->nstep = s->nstep; c->n_radimp_failures = s->n_fail; c->n_radimp_iters = s->n_iters;
+  c->n_radimp_solves = s->n_solves; c->n_guard_recoveries = s->n_guard;
+}
+void kzo_sim_set_time(kzo_sim *s, double t, double dtmax, double dtmin) {
+  s->t = t; s->tstepdenmax = dtmax; s->tstepdenmin = dtmin;
+}
+
+/* ---- polar axis (sim/polaraxis.zig) ---- */
+static int polar_row(const kzo_sim *s, long iy) {
+  if (!s->ph.correct_polaraxis) return 0;
+  long nc = s->ph.nccorrectpolar, ny = s->grid.ny;
+  return iy < nc || iy > ny - nc - 1;
+}
+
+/* ---- boundary conditions (sim/bc.zig + puffy.zig:1080-1194) ---- */
+static void tvec(const double *J, const double u[4], double out[4]) {
+  double r[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r[i] += J[i * 4 + j] * u[j];
+  for (int i = 0; i < 4; ++i) out[i] = r[i];
+}
+
+/* frames.zig transPmhdCocoJ */
+static int trans_pmhd(double *pp, const kzo_geo *g1, const kzo_geo *g2, const double *J) {
+  double u1c[4], u1v[4];
+  kzo_ucon_ucov_from_prims(pp, g1, u1c, u1v);
+  double B[3] = {pp[B1], pp[B2], pp[B3]};
+  double bcon[4], bcov[4], bsq;
+  kzo_bcon_bcov_bsq(B, u1c, u1v, g1, bcon, bcov, &bsq);
+  tvec(J, bcon, bcon);
+  double ucon[4];
+  tvec(J, u1c, ucon);
+  int e = kzo_convert(ucon, VEL4, VELR, g2, 1, ucon);
+  if (e) return e;
+  pp[VX] = ucon[1]; pp[VY] = ucon[2]; pp[VZ] = ucon[3];
+  double u2c[4], u2v[4];
+  kzo_ucon_ucov_from_prims(pp, g2, u2c, u2v);
+  for (int j = 1; j < 4; ++j) pp[B1 + j - 1] = bcon[j] * u2c[0] - bcon[0] * u2c[j];
+  return KZO_OK;
+}
+
+static void geo_from16(const double *g16, const double *G16, kzo_geo *o) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) { o->g[i][j] = g16[i * 4 + j]; o->G[i][j] = G16[i * 4 + j]; }
+  o->gdet = 0; o->alpha = kzm_sqrt(-1.0 / o->G[0][0]); o->gttpert = 0; o->x1 = 0;
+}
+
+/* puffy.zig:1080-1194 Bc.calc for one ghost cell: returns prims in pp */
+static int puffy_bc(kzo_sim *s, long ix, long iy, long iz, int face, double *pp) {
+  long nx = s->grid.nx, ny = s->grid.ny;
+  if (face == 1) { /* xhi */
+    memcpy(pp, cell(s->p, s, nx - 1, iy, iz), NV * sizeof(double));
+    kzo_geo geom, gbl;
+    geo_c(s, ix, iy, iz, &geom);
+    const double *rec = s->bcx + (((iz * (long)ny) + iy) * s->grid.ng + (ix - nx)) * BCX_REC;
+    geo_from16(rec, rec + 16, &gbl);
+    double rghost = rec[32], rbound = rec[33];
+    const double *J12 = rec + 34, *J21 = rec + 50;
+    int e = trans_pmhd(pp, &geom, &gbl, J12);
+    if (e) return e;
+    double scale1 = rbound * rbound / rghost / rghost;
+    double scale2 = rbound / rghost;
+    pp[RHO] *= scale1; pp[UU] *= scale1;
+    pp[B1] *= scale1; pp[B2] *= scale2; pp[B3] *= scale2;
+    pp[VY] *= scale1; pp[VZ] *= scale1;
+    pp[EE] *= scale1; pp[FY] *= scale1; pp[FZ] *= scale1;
+    e = trans_pmhd(pp, &gbl, &geom, J21);
+    if (e) return e;
+    double ucon[4] = {0.0, pp[VX], pp[VY], pp[VZ]};
+    e = kzo_convert(ucon, VELR, VEL4, &geom, 0, ucon);
+    if (e) return e;
+    tvec(J12, ucon, ucon);
+    if (ucon[1] < 0.0) {
+      ucon[1] = 0.0;
+      tvec(J21, ucon, ucon);
+      e = kzo_convert(ucon, VEL4, VELR, &geom, 0, ucon);
+      if (e) return e;
+      pp[VX] = ucon[1]; pp[VY] = ucon[2]; pp[VZ] = ucon[3];
+    }
+    double urf[4] = {0.0, pp[FX], pp[FY], pp[FZ]};
+    e = kzo_convert(urf, VELR, VEL4, &geom, 0, urf);
+    if (e) return e;
+    tvec(J12, urf, urf);
+    if (urf[1] < 0.0) {
+      urf[1] = 0.0;
+      tvec(J21, urf, urf);
+      e = kzo_convert(urf, VEL4, VELR, &geom, 0, urf);
+      if (e) return e;
+      pp[FX] = urf[1]; pp[FY] = urf[2]; pp[FZ] = urf[3];
+    }
+  } else if (face == 0) { /* xlo */
+    memcpy(pp, cell(s->p, s, 0, iy, iz), NV * sizeof(double));
+  } else if (face == 2) { /* ylo */
+    memcpy(pp, cell(s->p, s, ix, -iy - 1, iz), NV * sizeof(double));
+    pp[VY] = -pp[VY]; pp[B2] = -pp[B2]; pp[FY] = -pp[FY];
+  } else { /* yhi */
+    memcpy(pp, cell(s->p, s, ix, ny - (iy - ny) - 1, iz), NV * sizeof(double));
+    pp[VY] = -pp[VY]; pp[B2] = -pp[B2]; pp[FY] = -pp[FY];
+  }
+  return KZO_OK;
+}
+
+static int set_bc_cell(kzo_sim *s, long ix, long iy, long iz, int face) {
+  double pp[NV], uu[NV];
+  int e = puffy_bc(s, ix, iy, iz, face, pp);
+  if (e) return e;
+  kzo_geo g;
+  geo_c(s, ix, iy, iz, &g);
+  kzo_p2u(pp, uu, &g, s->ph.gam);
+  memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
+  memcpy(cell(s->u, s, ix, iy, iz), uu, sizeof uu);
+  return KZO_OK;
+}
+
+static void p2u_cell(kzo_sim *s, long ix, long iy, long iz) {
+  kzo_geo g;
+  geo_c(s, ix, iy, iz, &g);
+  kzo_p2u(cell(s->p, s, ix, iy, iz), cell(s->u, s, ix, iy, iz), &g, s->ph.gam);
+}
+static void copy_p(kzo_sim *s, long dx, long dy, long sx_, long sy_) {
+  memcpy(cell(s->p, s, dx, dy, 0), cell(s->p, s, sx_, sy_, 0), NV * sizeof(double));
+}
+static void avg_p(kzo_sim *s, long dx, long dy, long ax, long ay, long bx, long by) {
+  double *pa = cell(s->p, s, ax, ay, 0), *pb = cell(s->p, s, bx, by, 0), pp[NV];
+  for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
+  memcpy(cell(s->p, s, dx, dy, 0), pp, sizeof pp);
+}
+
+/* bc.zig:286-377 fillCorners2d (bc_x, bc_y not periodic) */
+static void fill_corners_2d(kzo_sim *s) {
+  long nx = s->grid.nx, ny = s->grid.ny, ng = s->grid.ng;
+  for (long i = 0; i < ng - 1; ++i) {
+    copy_p(s, -ng + i, -1, -ng + i, 0); p2u_cell(s, -ng + i, -1, 0);
+    copy_p(s, -1, -ng + i, 0, -ng + i); p2u_cell(s, -1, -ng + i, 0);
+  }
+  avg_p(s, -1, -1, -1, 0, 0, -1); p2u_cell(s, -1, -1, 0);
+  avg_p(s, -2, -2, -2, -1, -1, -2); p2u_cell(s, -2, -2, 0);
+  for (long i = 0; i < ng - 1; ++i) {
+    copy_p(s, -ng + i, ny, -ng + i, ny - 1); p2u_cell(s, -ng + i, ny, 0);
+    copy_p(s, -1, ny + i + 1, 0, ny + i + 1); p2u_cell(s, -1, ny + i + 1, 0);
+  }
+  avg_p(s, -1, ny, -1, ny - 1, 0, ny); p2u_cell(s, -1, ny, 0);
+  avg_p(s, -2, ny + 1, -2, ny, -1, ny + 1); p2u_cell(s, -2, ny + 1, 0);
+  for (long i = 0; i < ng - 1; ++i) {
+    copy_p(s, nx + i + 1, -1, nx + i + 1, 0); p2u_cell(s, nx + i + 1, -1, 0);
+    copy_p(s, nx, -ng + i, nx - 1, -ng + i); p2u_cell(s, nx, -ng + i, 0);
+  }
+  avg_p(s, nx, -1, nx - 1, -1, nx, 0); p2u_cell(s, nx, -1, 0);
+  avg_p(s, nx + 1, -2, nx, -2, nx + 1, -1); p2u_cell(s, nx + 1, -2, 0);
+  for (long i = 0; i < ng - 1; ++i) {
+    copy_p(s, nx + i + 1, ny, nx + i + 1, ny - 1); p2u_cell(s, nx + i + 1, ny, 0);
+    copy_p(s, nx, ny + i + 1, nx - 1, ny + i + 1); p2u_cell(s, nx, ny + i + 1, 0);
+  }
+  avg_p(s, nx, ny, nx - 1, ny, nx, ny - 1); p2u_cell(s, nx, ny, 0);
+  avg_p(s, nx + 1, ny + 1, nx, ny + 1, nx + 1, ny); p2u_cell(s, nx + 1, ny + 1, 0);
+}
+
+/* bc.zig:40-73 setBc */
+int kzo_sim_set_bc(kzo_sim *s, int ifinit) {
+  (void)ifinit;
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long i = 1; i <= s->ngx; ++i) {
+        int e = set_bc_cell(s, -i, iy, iz, 0);
+        if (e) return e;
+        e = set_bc_cell(s, nx - 1 + i, iy, iz, 1);
+        if (e) return e;
+      }
+  for (long iz = 0; iz < nz; ++iz)
+    for (long ix = 0; ix < nx; ++ix)
+      for (long i = 1; i <= s->ngy; ++i) {
+        int e = set_bc_cell(s, ix, -i, iz, 2);
+        if (e) return e;
+        e = set_bc_cell(s, ix, ny - 1 + i, iz, 3);
+        if (e) return e;
+      }
+  fill_corners_2d(s);
+  return KZO_OK;
+}
+
+/* ---- p2u over the domain (Sim.initCell's p2u) ---- */
+void kzo_sim_p2u_domain(kzo_sim *s) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) p2u_cell(s, ix, iy, 0);
+}
+
+/* ---- wavespeeds (sim.zig:747-886) ---- */
+static void save_wavespeeds(kzo_sim *s, long ix, long iy, long iz, const double aaa[6], const double rad0[6],
+                            const double radl[6], double *tmax, double *tmin) {
+  for (int d = 0; d < 3; ++d) {
+    *scp(s, ahd_l[d], ix, iy, iz) = aaa[2 * d];
+    *scp(s, ahd_r[d], ix, iy, iz) = aaa[2 * d + 1];
+  }
+  double ax = kzm_fmax(kzm_fabs(aaa[0]), kzm_fabs(aaa[1]));
+  double ay = kzm_fmax(kzm_fabs(aaa[2]), kzm_fabs(aaa[3]));
+  double az = kzm_fmax(kzm_fabs(aaa[4]), kzm_fabs(aaa[5]));
+  *scp(s, S_AHDX, ix, iy, iz) = ax;
+  *scp(s, S_AHDY, ix, iy, iz) = ay;
+  *scp(s, S_AHDZ, ix, iy, iz) = az;
+  for (int d = 0; d < 3; ++d) {
+    *scp(s, arad_l[d], ix, iy, iz) = radl[2 * d];
+    *scp(s, arad_r[d], ix, iy, iz) = radl[2 * d + 1];
+    *scp(s, arad_m[d], ix, iy, iz) = kzm_fmax(kzm_fabs(radl[2 * d]), kzm_fabs(radl[2 * d + 1]));
+  }
+  double wsx = kzm_fmax(ax, kzm_fmax(kzm_fabs(rad0[0]), kzm_fabs(rad0[1])));
+  double wsy = kzm_fmax(ay, kzm_fmax(kzm_fabs(rad0[2]), kzm_fabs(rad0[3])));
+  double wsz = kzm_fmax(az, kzm_fmax(kzm_fabs(rad0[4]), kzm_fabs(rad0[5])));
+  (void)wsz;
+  if (!(ix >= 0 && ix < s->grid.nx && iy >= 0 && iy < s->grid.ny && iz >= 0 && iz < s->grid.nz)) return;
+  double dx = csize(s, ix, 0), dy = csize(s, iy, 1);
+  double tsd = wsx / dx + wsy / dy; /* nz == 1, ny > 1 */
+  tsd /= s->ph.tsteplim;
+  *scp(s, S_TSTEPDEN, ix, iy, iz) = tsd;
+  if (tsd > *tmax) *tmax = tsd;
+  if (tsd < *tmin) *tmin = tsd;
+}
+
+static int calc_wavespeeds(kzo_sim *s) {
+  int active[3] = {s->grid.nx > 1, s->grid.ny > 1, s->grid.nz > 1};
+  long lx = active[0] ? 1 : 0, ly = active[1] ? 1 : 0, lz = active[2] ? 1 : 0;
+  double tmax = -1.0 / 0.0, tmin = 1.0 / 0.0;
+  for (long iz = -lz; iz < s->grid.nz + lz; ++iz)
+    for (long iy = -ly; iy < s->grid.ny + ly; ++iy)
+      for (long ix = -lx; ix < s->grid.nx + lx; ++ix) {
+        const double *pp = cell(s->p, s, ix, iy, iz);
+        kzo_geo g;
+        geo_c(s, ix, iy, iz, &g);
+        double aaa[6], rad0[6] = {0}, radl[6] = {0}, aval[12];
+        kzo_gas_wavespeeds_lr(pp, &g, s->ph.gam, active, aaa);
+        double tautot[3] = {0, 0, 0};
+        if (s->ph.has_opac) {
+          double chi = kzo_calc_chi_slim(pp, &g, s->ph.gam, &s->ph.opac);
+          long idx3[3] = {ix, iy, iz};
+          for (int d = 0; d < 3; ++d) {
+            double sg = kzm_sqrt(g.g[d + 1][d + 1]);
+            double dxp = kzm_fmax(csize(s, idx3[d], d) * sg, csize(s, idx3[d] + 1, d) * sg);
+            tautot[d] = chi * dxp;
+          }
+        }
+        if (s->ph.dampradwavespeednearaxis > 0) {
+          long nc = s->ph.dampradwavespeednearaxis;
+          if (iy < nc || iy >= s->grid.ny - nc) tautot[0] = tautot[1] = tautot[2] = 0;
+        }
+        kzo_calc_rad_wavespeeds(pp, &g, tautot, active, aval);
+        for (int i = 0; i < 6; ++i) { rad0[i] = aval[i]; radl[i] = aval[6 + i]; }
+        for (int d = 0; d < 3; ++d) {
+          if (radl[2 * d] > 0.0) radl[2 * d] = 0.0;
+          if (radl[2 * d + 1] < 0.0) radl[2 * d + 1] = 0.0;
+          if (rad0[2 * d] > 0.0) rad0[2 * d] = 0.0;
+          if (rad0[2 * d + 1] < 0.0) rad0[2 * d + 1] = 0.0;
+        }
+        save_wavespeeds(s, ix, iy, iz, aaa, rad0, radl, &tmax, &tmin);
+      }
+  if (tmax > s->tstepdenmax) s->tstepdenmax = tmax;
+  if (tmin < s->tstepdenmin) s->tstepdenmin = tmin;
+  return KZO_OK;
+}
+
+/* sim.zig:693-718 initTimestepGuess (2D) */
+void kzo_sim_init_timestep_guess(kzo_sim *s) {
+  const double ws = 10000.0;
+  double tsd = ws / s->min_dx + ws / s->min_dy;
+  tsd /= s->ph.tsteplim;
+  s->tstepdenmax = tsd;
+  s->tstepdenmin = tsd;
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      *scp(s, S_TSTEPDEN, ix, iy, 0) = tsd;
+      *scp(s, S_CELL_DT, ix, iy, 0) = 1.0 / tsd;
+    }
+}
+
+static void save_timesteps(kzo_sim *s) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix)
+      *scp(s, S_CELL_DT, ix, iy, 0) = 1.0 / *scp(s, S_TSTEPDEN, ix, iy, 0);
+}
+
+/* ---- u2p pass + fixups (sim.zig:924-1133) ---- */
+static void u2p_rows(kzo_sim *s) {
+  for (long iy = 0; iy < s->grid.ny; ++iy) {
+    int polar = polar_row(s, iy);
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      double uu[NV], pp[NV];
+      memcpy(uu, cell(s->u, s, ix, iy, 0), sizeof uu);
+      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+      *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0) = 0;
+      *flagp(s, KZ_FLAG_ENTROPY2, ix, iy, 0) = 0;
+      kzo_geo g;
+      geo_c(s, ix, iy, 0, &g);
+      if (polar) {
+        double gi = 1.0 / g.gdet;
+        pp[B1] = uu[B1] * gi; pp[B2] = uu[B2] * gi; pp[B3] = uu[B3] * gi;
+        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
+        *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, 0) = 0;
+        *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, 0) = 0;
+        continue;
+      }
+      kz_u2p_result r = kzo_u2p_mhd(uu, pp, &g, s->ph.gam, &s->ph.floors);
+      if (r.entropy_used) *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0) = 1;
+      kz_u2prad_result rr = kzo_u2p_rad(uu, pp, &g, &s->ph.rad);
+      kzo_check_floors_mhd(pp, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+      kzo_check_floors_rad(pp, &g, &s->ph.rad);
+      memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
+      *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, 0) = r.fixup ? 1 : 0;
+      *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, 0) = rr.corrected ? -1 : 0;
+    }
+  }
+}
+
+static void cell_fixup(kzo_sim *s, int which) {
+  int enabled = which == KZ_FLAG_HD_FIXUP ? s->ph.do_fixups
+                : which == KZ_FLAG_RAD_FIXUP ? (s->ph.do_fixups && s->ph.do_u2prad_fixups)
+                                             : (s->ph.do_fixups && s->ph.do_radimp_fixups);
+  if (!enabled) return;
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  int any = 0;
+  for (long iy = 0; iy < ny && !any; ++iy)
+    for (long ix = 0; ix < nx; ++ix)
+      if (*flagp(s, which, ix, iy, 0) != 0) { any = 1; break; }
+  if (!any) return;
+  size_t fb = (size_t)s->ncell * NV * sizeof(double);
+  memcpy(s->u_bak, s->u, fb);
+  memcpy(s->p_bak, s->p, fb);
+  for (long iy = 0; iy < ny; ++iy) {
+    if (polar_row(s, iy)) continue;
+    for (long ix = 0; ix < nx; ++ix) {
+      if (*flagp(s, which, ix, iy, 0) == 0) continue;
+      double ppn[6][NV];
+      int in_n = 0;
+      long nb[6][3] = {{ix - 1, iy, 0}, {ix + 1, iy, 0}, {ix, iy - 1, 0}, {ix, iy + 1, 0}, {ix, iy, -1}, {ix, iy, 1}};
+      for (int k = 0; k < 6; ++k) {
+        if (nb[k][0] < 0 || nb[k][0] >= nx || nb[k][1] < 0 || nb[k][1] >= ny || nb[k][2] < 0 || nb[k][2] >= nz) continue;
+        if (*flagp(s, which, nb[k][0], nb[k][1], nb[k][2]) != 0) continue;
+        memcpy(ppn[in_n], cell(s->p, s, nb[k][0], nb[k][1], nb[k][2]), NV * sizeof(double));
+        in_n += 1;
+      }
+      int enough = (nz == 1 && ny == 1 && in_n >= 1) || (nz == 1 && in_n >= 2) || (ny == 1 && in_n >= 2) || in_n >= 3;
+      if (!enough) continue;
+      double pp[NV], uu[NV];
+      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+      for (int iv = 0; iv < NV; ++iv) {
+        int fixthis;
+        if (which == KZ_FLAG_HD_FIXUP) fixthis = iv != RHO && iv < B1;
+        else if (which == KZ_FLAG_RAD_FIXUP) fixthis = iv >= EE && iv <= FZ;
+        else fixthis = iv != RHO && (iv < B1 || (iv >= EE && iv <= FZ));
+        if (fixthis) {
+          double sum = 0;
+          for (int k = 0; k < in_n; ++k) sum += ppn[k][iv];
+          pp[iv] = sum / (double)in_n;
+        }
+      }
+      kzo_geo g;
+      geo_c(s, ix, iy, 0, &g);
+      kzo_p2u(pp, uu, &g, s->ph.gam);
+      memcpy(cell(s->u_bak, s, ix, iy, 0), uu, sizeof uu);
+      memcpy(cell(s->p_bak, s, ix, iy, 0), pp, sizeof pp);
+    }
+  }
+  memcpy(s->u, s->u_bak, fb);
+  memcpy(s->p, s->p_bak, fb);
+}
+
+int kzo_sim_calc_u2p(kzo_sim *s) {
+  u2p_rows(s);
+  cell_fixup(s, KZ_FLAG_HD_FIXUP);
+  cell_fixup(s, KZ_FLAG_RAD_FIXUP);
+  return kzo_sim_set_bc(s, 0);
+}
+
+/* ---- metric source (sim.zig:1140-1175) ---- */
+static void metric_source(kzo_sim *s, long ix, long iy, long iz, double ss[NV]) {
+  kzo_geo g;
+  geo_c(s, ix, iy, iz, &g);
+  const double *pp = cell(s->p, s, ix, iy, iz);
+  double gdetu = g.gdet;
+  for (int i = 0; i < NV; ++i) ss[i] = 0;
+  double tij[4][4], t[4][4], rup[4][4], rij[4][4];
+  kzo_calc_tij(pp, &g, s->ph.gam, tij);
+  kzo_lower_second(tij, &g, t);
+  const double *kr = s->kris + cidx(s, ix, iy, iz) * 64;
+  static const int rows[4] = {UU, VX, VY, VZ};
+  static const int rrows[4] = {EE, FX, FY, FZ};
+  kzo_calc_rij(pp, &g, rup);
+  kzo_lower_second(rup, &g, rij);
+  for (int k = 0; k < 4; ++k)
+    for (int l = 0; l < 4; ++l)
+      for (int nu = 0; nu < 4; ++nu) {
+        ss[rows[nu]] += gdetu * t[k][l] * kr[l * 16 + nu * 4 + k];
+        ss[rrows[nu]] += gdetu * rij[k][l] * kr[l * 16 + nu * 4 + k];
+      }
+}
+
+/* ---- explicit operator (sim.zig:1209-1582) ---- */
+static long cross_lo(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? -1 : 0; }
+static long cross_hi(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? ndim(s, dim) + 1 : ndim(s, dim); }
+
+static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
+  long n = ndim(s, dim);
+  int dol = i >= 0, dor = i < n;
+  double st[5][NV];
+  long c[3] = {cellc[0], cellc[1], cellc[2]};
+  for (int k = 0; k < 5; ++k) {
+    c[dim] = i - 2 + k;
+    memcpy(st[k], cell(s->p, s, c[0], c[1], c[2]), NV * sizeof(double));
+  }
+  double dx5[5];
+  for (int k = 0; k < 5; ++k) dx5[k] = csize(s, i - 2 + k, dim);
+  double pl[NV], pr[NV];
+  for (int iv = 0; iv < NV; ++iv) {
+    double u5[5] = {st[0][iv], st[1][iv], st[2][iv], st[3][iv], st[4][iv]};
+    kzo_reconstruct(2, s->ph.minmod_theta, u5, dx5, &pl[iv], &pr[iv]);
+  }
+  double ffl[NV], ffr[NV];
+  if (dol) {
+    kzo_geo g;
+    geo_f(s, cellc[0], cellc[1], cellc[2], dim, &g);
+    kzo_check_floors_mhd(pl, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+    int e = kzo_f_flux_prime(pl, dim, &g, s->ph.gam, ffl);
+    if (e) return e;
+  }
+  long cp[3] = {cellc[0], cellc[1], cellc[2]};
+  cp[dim] = i + 1;
+  if (dor) {
+    kzo_geo g;
+    geo_f(s, cp[0], cp[1], cp[2], dim, &g);
+    kzo_check_floors_mhd(pr, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+    int e = kzo_f_flux_prime(pr, dim, &g, s->ph.gam, ffr);
+    if (e) return e;
+  }
+  memcpy(face(s->pb_r[dim], s, dim, cellc[0], cellc[1], cellc[2]), pl, sizeof pl);
+  if (dol) memcpy(face(s->fl_r[dim], s, dim, cellc[0], cellc[1], cellc[2]), ffl, sizeof ffl);
+  memcpy(face(s->pb_l[dim], s, dim, cp[0], cp[1], cp[2]), pr, sizeof pr);
+  if (dor) memcpy(face(s->fl_l[dim], s, dim, cp[0], cp[1], cp[2]), ffr, sizeof ffr);
+  return KZO_OK;
+}
+
+static int sweep(kzo_sim *s, int dim) {
+  long n = ndim(s, dim);
+  if (n <= 1) return KZO_OK;
+  int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
+  for (long c0 = cross_lo(s, c0d); c0 < cross_hi(s, c0d); ++c0)
+    for (long c1 = cross_lo(s, c1d); c1 < cross_hi(s, c1d); ++c1)
+      for (long i = -1; i < n + 1; ++i) {
+        long cl[3];
+        cl[dim] = i; cl[c0d] = c0; cl[c1d] = c1;
+        int e = sweep_face(s, dim, i, cl);
+        if (e) return e;
+      }
+  return KZO_OK;
+}
+
+static int flux_face(kzo_sim *s, int dim, const long cf[3]) {
+  long cm[3] = {cf[0], cf[1], cf[2]};
+  cm[dim] -= 1;
+  double ap1 = *scp(s, ahd_m[dim], cf[0], cf[1], cf[2]);
+  double am1 = *scp(s, ahd_m[dim], cm[0], cm[1], cm[2]);
+  double pLl[NV], pRl[NV], uLl[NV], uRl[NV];
+  memcpy(pLl, face(s->pb_l[dim], s, dim, cf[0], cf[1], cf[2]), sizeof pLl);
+  memcpy(pRl, face(s->pb_r[dim], s, dim, cf[0], cf[1], cf[2]), sizeof pRl);
+  kzo_geo g;
+  geo_f(s, cf[0], cf[1], cf[2], dim, &g);
+  kzo_p2u(pLl, uLl, &g, s->ph.gam);
+  kzo_p2u(pRl, uRl, &g, s->ph.gam);
+  double ag = kzm_fmax(ap1, am1);
+  double rp1 = *scp(s, arad_m[dim], cf[0], cf[1], cf[2]);
+  double rm1 = *scp(s, arad_m[dim], cm[0], cm[1], cm[2]);
+  double agr = kzm_fmax(rp1, rm1);
+  double *fll = face(s->fl_l[dim], s, dim, cf[0], cf[1], cf[2]);
+  double *flr = face(s->fl_r[dim], s, dim, cf[0], cf[1], cf[2]);
+  double fstar[NV];
+  for (int iv = 0; iv < NV; ++iv) fstar[iv] = kzo_laxf(fll[iv], flr[iv], uLl[iv], uRl[iv], iv >= EE ? agr : ag);
+  memcpy(face(s->flb[dim], s, dim, cf[0], cf[1], cf[2]), fstar, sizeof fstar);
+  return KZO_OK;
+}
+
+static int fluxes_at_faces(kzo_sim *s) {
+  for (int d = 0; d < 3; ++d) memset(s->flb[d], 0, (size_t)s->nface[d] * NV * sizeof(double));
+  for (int dim = 0; dim < 3; ++dim) {
+    long n = ndim(s, dim);
+    if (n <= 1) continue;
+    int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
+    for (long c0 = cross_lo(s, c0d); c0 < cross_hi(s, c0d); ++c0)
+      for (long c1 = cross_lo(s, c1d); c1 < cross_hi(s, c1d); ++c1)
+        for (long i = 0; i <= n; ++i) {
+          long cf[3];
+          cf[dim] = i; cf[c0d] = c0; cf[c1d] = c1;
+          flux_face(s, dim, cf);
+        }
+  }
+  return KZO_OK;
+}
+
+/* ct.zig:38-177 fluxCt */
+static long emf_idx(const kzo_sim *s, long ix, long iy, long iz) {
+  return (iz * (s->grid.ny + 1) + iy) * (long)(s->grid.nx + 1) + ix;
+}
+static long pin(long n, long i) { return n == 1 ? 0 : i; }
+static double flbg(kzo_sim *s, int d, int iv, long ix, long iy, long iz) { return face(s->flb[d], s, d, ix, iy, iz)[iv]; }
+
+void kzo_flux_ct(kzo_sim *s) {
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  double coef1 = (ny > 1 && nz > 1) ? 0.25 : 0.5;
+  double coef2 = (nx > 1 && nz > 1) ? 0.25 : 0.5;
+  double coef3 = (nx > 1 && ny > 1) ? 0.25 : 0.5;
+  for (long iz = 0; iz <= nz; ++iz)
+    for (long iy = 0; iy <= ny; ++iy)
+      for (long ix = 0; ix <= nx; ++ix) {
+        long e = emf_idx(s, ix, iy, iz);
+        if (ny > 1 || nz > 1) {
+          double sum = 0;
+          if (ny > 1) sum += flbg(s, 1, B3, pin(nx, ix), pin(ny, iy), pin(nz, iz)) + flbg(s, 1, B3, pin(nx, ix), pin(ny, iy), pin(nz, iz - 1));
+          if (nz > 1) sum -= flbg(s, 2, B2, pin(nx, ix), pin(ny, iy), pin(nz, iz)) + flbg(s, 2, B2, pin(nx, ix), pin(ny, iy - 1), pin(nz, iz));
+          s->emf[0][e] = coef1 * sum;
+        } else s->emf[0][e] = 0;
+        if (nx > 1 || nz > 1) {
+          double sum = 0;
+          if (nz > 1) sum += flbg(s, 2, B1, pin(nx, ix), pin(ny, iy), pin(nz, iz)) + flbg(s, 2, B1, pin(nx, ix - 1), pin(ny, iy), pin(nz, iz));
+          if (nx > 1) sum -= flbg(s, 0, B3, ix, pin(ny, iy), pin(nz, iz)) + flbg(s, 0, B3, ix, pin(ny, iy), pin(nz, iz - 1));
+          s->emf[1][e] = coef2 * sum;
+        } else s->emf[1][e] = 0;
+        if (nx > 1 || ny > 1) {
+          double sum = 0;
+          if (nx > 1) sum += flbg(s, 0, B2, ix, pin(ny, iy), pin(nz, iz)) + flbg(s, 0, B2, ix, pin(ny, iy - 1), pin(nz, iz));
+          if (ny > 1) sum -= flbg(s, 1, B1, pin(nx, ix), pin(ny, iy), pin(nz, iz)) + flbg(s, 1, B1, pin(nx, ix - 1), pin(ny, iy), pin(nz, iz));
+          s->emf[2][e] = coef3 * sum;
+        } else s->emf[2][e] = 0;
+      }
+  for (long iz = 0; iz <= nz; ++iz)
+    for (long iy = 0; iy <= ny; ++iy)
+      for (long ix = 0; ix <= nx; ++ix) {
+#define EMF(c, a, b, d) s->emf[(c)-1][emf_idx(s, a, b, d)]
+        if (nx > 1 && iy < ny && iz < nz) {
+          double *f = face(s->flb[0], s, 0, ix, iy, iz);
+          f[B1] = 0;
+          f[B2] = 0.5 * (EMF(3, ix, iy, iz) + EMF(3, ix, iy + 1, iz));
+          f[B3] = -0.5 * (EMF(2, ix, iy, iz) + EMF(2, ix, iy, iz + 1));
+        }
+        if (ny > 1 && ix < nx && iz < nz) {
+          double *f = face(s->flb[1], s, 1, ix, iy, iz);
+          f[B1] = -0.5 * (EMF(3, ix, iy, iz) + EMF(3, ix + 1, iy, iz));
+          f[B2] = 0;
+          f[B3] = 0.5 * (EMF(1, ix, iy, iz) + EMF(1, ix, iy, iz + 1));
+        }
+        if (nz > 1 && ix < nx && iy < ny) {
+          double *f = face(s->flb[2], s, 2, ix, iy, iz);
+          f[B1] = 0.5 * (EMF(2, ix, iy, iz) + EMF(2, ix + 1, iy, iz));
+          f[B2] = -0.5 * (EMF(1, ix, iy, iz) + EMF(1, ix, iy + 1, iz));
+          f[B3] = 0;
+        }
+#undef EMF
+      }
+}
+
+static void update_rows(kzo_sim *s, double dtin) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      double ms[NV];
+      metric_source(s, ix, iy, 0, ms);
+      double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, 0, 2);
+      double dt = dtin;
+      double *flxl = face(s->flb[0], s, 0, ix, iy, 0), *flxr = face(s->flb[0], s, 0, ix + 1, iy, 0);
+      double *flyl = face(s->flb[1], s, 1, ix, iy, 0), *flyr = face(s->flb[1], s, 1, ix, iy + 1, 0);
+      /* z faces of a collapsed dim: the store is zeroed by fluxesAtFaces */
+      double *flzl = face(s->flb[2], s, 2, ix, iy, 0), *flzr = face(s->flb[2], s, 2, ix, iy, 1);
+      double *uu = cell(s->u, s, ix, iy, 0);
+      for (int iv = 0; iv < NV; ++iv) {
+        double du = -(flxr[iv] - flxl[iv]) * dt / dx - (flyr[iv] - flyl[iv]) * dt / dy - (flzr[iv] - flzl[iv]) * dt / dz;
+        uu[iv] = uu[iv] + du + ms[iv] * dt;
+      }
+    }
+}
+
+int kzo_sim_op_explicit(kzo_sim *s, double dt) {
+  int e = calc_wavespeeds(s);
+  if (e) return e;
+  for (int dim = 0; dim < 3; ++dim) {
+    e = sweep(s, dim);
+    if (e) return e;
+  }
+  fluxes_at_faces(s);
+  kzo_flux_ct(s);
+  update_rows(s, dt);
+  return kzo_sim_calc_u2p(s);
+}
+
+/* ---- implicit operator (sim.zig:1715-1776) ---- */
+int kzo_sim_op_implicit(kzo_sim *s, double dtin) {
+  if (!s->ph.has_opac) return KZO_OK;
+  for (long iy = 0; iy < s->grid.ny; ++iy) {
+    if (polar_row(s, iy)) continue;
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      double uu[NV], pp[NV];
+      memcpy(uu, cell(s->u, s, ix, iy, 0), sizeof uu);
+      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+      kzo_geo g;
+      geo_c(s, ix, iy, 0, &g);
+      kz_implicit_result r = kzo_solve_implicit_lab(uu, pp, &g, dtin, s->ph.gam, &s->ph.rad, &s->ph.opac, &s->ph.implicit);
+      if (r.ok) {
+        memcpy(cell(s->u, s, ix, iy, 0), uu, sizeof uu);
+        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
+        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, 0) = 0;
+        s->n_iters += (uint64_t)r.iters;
+        s->n_solves += 1;
+      } else {
+        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, 0) = -1;
+        s->n_fail += 1;
+      }
+    }
+  }
+  cell_fixup(s, KZ_FLAG_RADIMP_FIXUP);
+  return KZO_OK;
+}
+
+/* ---- polar axis overwrite (polaraxis.zig:84-147) ---- */
+int kzo_sim_do_correct(kzo_sim *s) {
+  if (!s->ph.correct_polaraxis) return KZO_OK;
+  long nc = s->ph.nccorrectpolar, ny = s->grid.ny;
+  for (long ix = 0; ix < s->grid.nx; ++ix)
+    for (long ic = 0; ic < nc; ++ic) {
+      long rows2[2] = {ic, ny - 1 - ic};
+      for (int w = 0; w < 2; ++w) {
+        long iy = rows2[w];
+        long iysrc = iy < nc ? nc : ny - 1 - nc;
+        double thaxis = yl_(s, iy < nc ? 0 : ny);
+        double fac = kzm_fabs((yc_(s, iy) - thaxis) / (yc_(s, iysrc) - thaxis));
+        double pp[NV], uu[NV];
+        double *ps = cell(s->p, s, ix, iysrc, 0);
+        memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+        pp[RHO] = ps[RHO]; pp[UU] = ps[UU]; pp[ENTR] = ps[ENTR];
+        pp[VX] = ps[VX]; pp[VZ] = ps[VZ]; pp[VY] = fac * ps[VY];
+        pp[EE] = ps[EE]; pp[FX] = ps[FX]; pp[FZ] = ps[FZ]; pp[FY] = fac * ps[FY];
+        kzo_geo g;
+        geo_c(s, ix, iy, 0, &g);
+        kzo_p2u(pp, uu, &g, s->ph.gam);
+        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
+        memcpy(cell(s->u, s, ix, iy, 0), uu, sizeof uu);
+      }
+    }
+  return KZO_OK;
+}
+
+/* ---- stage arithmetic (sim.zig:1586-1709) ---- */
+static void stage_deriv(kzo_sim *s, double *dst, const double *a, const double *b, double delta) {
+  double f1 = 1.0 / delta, f2 = -1.0 / delta;
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      long o = cidx(s, ix, iy, 0) * NV;
+      for (int iv = 0; iv < NV; ++iv) dst[o + iv] = f1 * a[o + iv] + f2 * b[o + iv];
+    }
+}
+static void stage_combine(kzo_sim *s, double *dst, const double *a, double f1, const double *b, double f2, const double *c) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      long o = cidx(s, ix, iy, 0) * NV;
+      for (int iv = 0; iv < NV; ++iv) dst[o + iv] = a[o + iv] + f1 * b[o + iv] + f2 * c[o + iv];
+    }
+}
+static void copy_full(kzo_sim *s, double *dst, const double *src) { memcpy(dst, src, (size_t)s->ncell * NV * sizeof(double)); }
+
+static void copy_entropy_count(kzo_sim *s) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix)
+      *flagp(s, KZ_FLAG_ENTROPY3, ix, iy, 0) = *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0);
+}
+
+static void update_entropy(kzo_sim *s) {
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      double *pp = cell(s->p, s, ix, iy, 0);
+      pp[ENTR] = kzo_s_from_u(pp[RHO], pp[UU], s->ph.gam);
+      kzo_geo g;
+      geo_c(s, ix, iy, 0, &g);
+      kzo_p2u_mhd(pp, cell(s->u, s, ix, iy, 0), &g, s->ph.gam);
+    }
+}
+
+/* The RK2IMEX gamma 1 - 1/sqrt(2), correctly rounded (sim.zig:1821 is a
+ * comptime expression evaluated above f64 precision). */
+static const double GAM_IMEX = 0x1.2bec333018867p-2;
+
+/* sim.zig:1792-1899 step */
+int kzo_sim_step(kzo_sim *s, const double *forced_dt) {
+  double t = s->t;
+  s->own_dt = 1.0 / s->tstepdenmax;
+  double dt = forced_dt ? *forced_dt : s->own_dt;
+  s->dt = dt;
+  s->tstepdenmax = -1;
+  s->tstepdenmin = 1.0e50;
+  const double g = GAM_IMEX;
+  int e;
+  save_timesteps(s);
+  copy_full(s, s->ut0, s->u);
+  if ((e = kzo_sim_op_implicit(s, dt * g))) return e;
+  stage_deriv(s, s->drt1, s->u, s->ut0, dt * g);
+
+  copy_full(s, s->ut1, s->u);
+  if ((e = kzo_sim_calc_u2p(s))) return e;
+  kzo_sim_do_correct(s);
+  if ((e = kzo_sim_set_bc(s, 0))) return e;
+  if ((e = kzo_sim_op_explicit(s, dt))) return e;
+  copy_entropy_count(s);
+  stage_deriv(s, s->dut1, s->u, s->ut1, dt);
+
+  stage_combine(s, s->u, s->ut0, dt, s->dut1, dt * (1.0 - 2.0 * g), s->drt1);
+
+  copy_full(s, s->uforget, s->u);
+  if ((e = kzo_sim_calc_u2p(s))) return e;
+  kzo_sim_do_correct(s);
+  if ((e = kzo_sim_op_implicit(s, g * dt))) return e;
+  stage_deriv(s, s->drt2, s->u, s->uforget, dt * g);
+
+  copy_full(s, s->ut2, s->u);
+  if ((e = kzo_sim_calc_u2p(s))) return e;
+  kzo_sim_do_correct(s);
+  if ((e = kzo_sim_set_bc(s, 0))) return e;
+  if ((e = kzo_sim_op_explicit(s, dt))) return e;
+  stage_deriv(s, s->dut2, s->u, s->ut2, dt);
+
+  stage_combine(s, s->u, s->ut0, dt / 2.0, s->dut1, dt / 2.0, s->dut2);
+  stage_combine(s, s->u, s->u, dt / 2.0, s->drt1, dt / 2.0, s->drt2);
+
+  if ((e = kzo_sim_calc_u2p(s))) return e;
+  kzo_sim_do_correct(s);
+  if ((e = kzo_sim_set_bc(s, 0))) return e;
+  s->t = t + dt;
+  update_entropy(s);
+  s->nstep += 1;
+  return KZO_OK;
+}
+
+/* puffy.zig:892-1024 postinit (BETANORMFULL): scale B by sqrt(maxbeta/max
+ * pmag/ptot) over the domain and refresh u. Ghosts keep the unscaled B. */
+double kzo_sim_postinit(kzo_sim *s, double maxbeta) {
+  double maxb = 0.0;
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      const double *pp = cell(s->p, s, ix, iy, 0);
+      kzo_geo g;
+      geo_c(s, ix, iy, 0, &g);
+      double ucon[4], ucov[4];
+      kzo_ucon_ucov_from_prims(pp, &g, ucon, ucov);
+      double B[3] = {pp[B1], pp[B2], pp[B3]}, bcon[4], bcov[4], bsq;
+      kzo_bcon_bcov_bsq(B, ucon, ucov, &g, bcon, bcov, &bsq);
+      double pmag = bsq / 2.0;
+      double pgas = (s->ph.gam - 1.0) * pp[UU];
+      double ptot = pgas;
+      double rtt = kzo_calc_ff_rtt(pp, &g, 0);
+      double ehat = -rtt;
+      ptot += ehat / 3.0;
+      if (pmag / ptot > maxb) maxb = pmag / ptot;
+    }
+  double fac = kzm_sqrt(maxbeta / maxb);
+  for (long iy = 0; iy < s->grid.ny; ++iy)
+    for (long ix = 0; ix < s->grid.nx; ++ix) {
+      double *pp = cell(s->p, s, ix, iy, 0);
+      pp[B1] *= fac; pp[B2] *= fac; pp[B3] *= fac;
+      p2u_cell(s, ix, iy, 0);
+    }
+  return fac;
+}
