@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""zone-cycles/s of the full RMHD+M1 step (PPM+LAXF, flux-CT, u2p, M1
+implicit four-force) of a 384x360 PUFFY torus on one MI355X.
+
+Workload: the seeded-perturbed PUFFY torus (M=10 Msun, spin 0, MKS2, 384x360
++ 3 ghosts, radiative viscosity and dynamo off), W warmup CFL steps, then K
+timed CFL steps enqueued back to back on the sim's stream with the timestep
+kept on the device.  The state is resident in HBM before the clock starts.
+
+Prints ONE JSON line (see the keys below).  Beside the throughput it reports
+  roofline      the dominant kernel class, timed live with HIP events on the
+                sim's stream over the timed region, against MI355X's peak;
+  cpu_baseline  the oracle (the CPU restatement, one thread) timed on this
+                host over a bounded sample of the same workload;
+  p50/p99 step latency over a short run of individually synchronised steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+SEED = 0x4B5A4D493335355F  # default perturbation seed of the reference's puffy runs
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic (compulsory) HBM bytes per launch unit of each HBM-bound kernel
+# class, 2D, FP64 (13 slots = 104 B per state):
+#  face_flux (per dim, per face): read p of the two cells' new stencil cell
+#    104 + face geometry g,G symmetric 20 + gdet/gttpert 2 -> 22*8 = 176
+#    + two wavespeeds 16 + write 13 fluxes 104                 = 400 B
+#  update (per cell): read u 104 + p 104 + centre geometry 176 + 40 unique
+#    Christoffels 320 + 4 face fluxes read 4*104 + write u 104  = 1224 B
+#  u2p (per cell): read u 104 + p 104 + geometry 176 + write p 104
+#    + 4 flags 16                                               = 504 B
+BYTES_PER_UNIT = {"face_flux": 400.0, "update": 1224.0, "u2p": 504.0}
+# FP64 flops of one implicit cell-solve: an a-priori operation count of the
+# solver's fixed part plus its per-Newton-iteration part (Jacobian by 4
+# residual differences, 4x4 inverse, update); refined by an instrumented
+# count later.
+IMPLICIT_FLOPS_PER_SOLVE = 12000.0
+IMPLICIT_FLOPS_PER_ITER = 10000.0
+
+
+def _pkg():
+    import importlib.util
+    name = "koral_zig_amd"
+    if name not in sys.modules:
+        pkg = ROOT / "koral-zig_amd"
+        spec = importlib.util.spec_from_file_location(name, pkg / "__init__.py", submodule_search_locations=[str(pkg)])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[name] = mod
+        spec.loader.exec_module(mod)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(grid, phys, tab, p, u, clock, steps):
+    """The oracle (single thread) over `steps` CFL steps from the GPU's
+    post-warmup state: zone-cycles/s on this host."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import kzo
+    sim = kzo.Sim(grid, phys, tab)
+    sim.set_state(p, u)
+    sim.set_time(clock["t"], clock["tstepdenmax"], clock["tstepdenmin"])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    return grid.nx * grid.ny * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=384)
+    ap.add_argument("--ny", type=int, default=360)
+    ap.add_argument("--seed", type=int, default=SEED)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--latency-steps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs the step on one GPU only; --gpus must be 1 "
+                                   "(the multi-GPU phi-slab decomposition is a design, not built)"}))
+        sys.exit(2)
+
+    _pkg()
+    from koral_zig_amd import abi_types as T
+    from koral_zig_amd import hip, host
+
+    grid = T.grid_puffy(args.nx, args.ny)
+    phys = T.physics_puffy()
+    t0 = time.perf_counter()
+    tab = host.metric_tables(grid)
+    sim = hip.Sim(grid, phys, tab)
+    host.init_puffy(sim, grid, tab, phys, seed=args.seed)
+    log(f"setup {time.perf_counter() - t0:.1f}s (metric tables, PUFFY init, perturbation seed {args.seed:#x})")
+
+    sim.step_many(args.warmup)
+    sim.sync()
+    log(f"warmup {args.warmup} steps done")
+
+    # per-class breakdown over one step each (outside the timed region)
+    classes = ["face_flux", "implicit", "u2p", "update", "wavespeeds", "ct", "bc"]
+    per_step = {}
+    for k in classes:
+        sim.time_kernel(k)
+        sim.step_many(1)
+        ms, n = sim.kernel_time()
+        per_step[k] = ms
+    sim.time_kernel(None)
+    dom = max(per_step, key=per_step.get)
+    log("per-step kernel ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per_step.items()) + f" -> dominant {dom}")
+
+    c0 = sim.counters()
+    sim.time_kernel(dom)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_many(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_dom, n_dom = sim.kernel_time()
+    sim.time_kernel(None)
+    c1 = sim.counters()
+    zones = grid.nx * grid.ny
+    value = zones * args.steps / wall
+    log(f"timed {args.steps} steps in {wall:.3f}s: {value:.4e} zone-cycles/s; t={c1['t']:.6e}")
+
+    avg_ms = ms_dom / max(n_dom, 1)
+    if dom in BYTES_PER_UNIT:
+        if dom == "face_flux":
+            units = (grid.nx + 1) * (grid.ny + 2) + (grid.ny + 1) * (grid.nx + 2)  # both dims, per pair
+            units_per_launch = units / 2.0
+        else:
+            units_per_launch = zones
+        achieved = BYTES_PER_UNIT[dom] * units_per_launch / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    else:
+        solves = c1["n_radimp_solves"] - c0["n_radimp_solves"]
+        iters = c1["n_radimp_iters"] - c0["n_radimp_iters"]
+        launches = 2 * args.steps if dom == "implicit" else n_dom
+        flops = (IMPLICIT_FLOPS_PER_SOLVE * solves + IMPLICIT_FLOPS_PER_ITER * iters) / max(launches, 1)
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches"] = n_dom
+
+    # step latency: individually synchronised steps
+    lat = []
+    for _ in range(args.latency_steps):
+        a = time.perf_counter()
+        sim.step_many(1)
+        sim.sync()
+        lat.append((time.perf_counter() - a) * 1e3)
+    lat = np.array(lat)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        p, u, _ = sim.get_state()
+        v, secs = cpu_baseline(grid, phys, tab, p, u, sim.counters(), args.cpu_steps)
+        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} CFL steps of the same {args.nx}x{args.ny} run from the GPU's "
+                         f"post-warmup state ({secs:.1f}s, one thread)"}
+        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+    sim.close()
+
+    out = {
+        "metric": "zone-cycles/s full RMHD+M1 step at 384x360 on 1 GPU; p50/p99 step latency batched",
+        "value": value,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"PUFFY torus {args.nx}x{args.ny}x1 MKS2 a=0 M=10Msun, seeded 1e-4 perturbation, "
+                               f"CFL steps, radviscosity/dynamo off",
+                   "nx": args.nx, "ny": args.ny, "seed": args.seed},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "step_latency_ms": ({"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                             "n": int(lat.size)} if lat.size else None),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+/* kz_step.h — the koral per-step hot path on one MI355X (libkz_hip.so).
+ *
+ * A kz_sim holds one run's device-resident state: primitives, conserved
+ * variables, the RK2IMEX stage copies, face fluxes, EMFs, per-cell flags
+ * and the metric tables, all in HBM.  Each entry point replaces one method
+ * of the reference's Sim(config.puffy) (koral/sim.zig:93-1900); the caller
+ * keeps the reference's array formats at this boundary (padded AoS fields
+ * iv-fastest, koral/field.zig; MetricCache 20-double blocks,
+ * koral/metric/precompute.zig:121-526) and the library converts to its own
+ * SoA device layout inside.
+ *
+ * Return codes are the KZ_OK / KZ_E_* values of kz_types.h.  Calls on one
+ * kz_sim are not thread-safe; different kz_sim handles are independent.
+ */
+#ifndef KZ_STEP_H
+#define KZ_STEP_H
+
+#include <stdint.h>
+#include "kz_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kz_sim kz_sim;
+
+/* Sim.init (koral/sim.zig:309-500) with the MetricCache blocks computed by
+ * the caller (kz_host_metric_tables) and the PUFFY outer-boundary table
+ * (kz_host_bc_xhi_tables). Returns NULL and sets *err on failure:
+ * KZ_E_UNSUPPORTED for nz != 1 or an option outside this build
+ * (radviscosity, dynamo, reduce-order, synchrotron bridge, ZAMO floors),
+ * KZ_E_INVALID_CONFIG for sim.zig:319-363's preconditions. */
+kz_sim *kz_sim_create(const kz_grid *grid, const kz_physics *phys, const double *g, const double *gcon,
+                      const double *kris, const double *gb0, const double *gconb0, const double *gb1,
+                      const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err);
+void kz_sim_destroy(kz_sim *s);
+
+/* State exchange in the reference Field layout (padded AoS, 13 slots per
+ * cell; flags 6 int32 per cell). NULL pointers are skipped. Synchronous. */
+int kz_sim_set_state(kz_sim *s, const double *p, const double *u);
+int kz_sim_get_state(kz_sim *s, double *p, double *u, int32_t *flags);
+
+/* Sim.initCell's p2u over the domain (koral/sim.zig:672-685). */
+int kz_sim_p2u_domain(kz_sim *s);
+/* Sim.setBc (koral/sim.zig:735-745 -> sim/bc.zig:40-73, PUFFY Bc.calc
+ * koral/problems/puffy/puffy.zig:1080-1194, fillCorners2d bc.zig:286-377). */
+int kz_sim_set_bc(kz_sim *s, int ifinit);
+/* Physics.postinit beta normalisation (puffy.zig:892-1024); *fac_out gets
+ * the B scale factor. */
+int kz_sim_postinit(kz_sim *s, double maxbeta, double *fac_out);
+/* Sim.initTimestepGuess (koral/sim.zig:693-718). */
+int kz_sim_init_timestep_guess(kz_sim *s);
+/* Restore the step clock (t, tstepdenmax, tstepdenmin) of a resumed run. */
+int kz_sim_set_time(kz_sim *s, double t, double tstepdenmax, double tstepdenmin);
+
+/* The passes of one step, each synchronous:
+ * Sim.calcU2p (koral/sim.zig:924-1026, with cellFixup :1027-1133 and setBc),
+ * Sim.opExplicit (koral/sim.zig:1506-1582: calcWavespeeds :747-886,
+ *   sweeps + fluxesAtFaces :1209-1503, sim/ct.zig:38-177 fluxCt, update),
+ * Sim.opImplicit (koral/sim.zig:1715-1776),
+ * Sim.doCorrect (koral/sim.zig:1783-1790 -> sim/polaraxis.zig:84-147). */
+int kz_calc_u2p(kz_sim *s);
+int kz_op_explicit(kz_sim *s, double dt);
+int kz_op_implicit(kz_sim *s, double dt);
+int kz_do_correct(kz_sim *s);
+
+/* Sim.step (koral/sim.zig:1792-1899): one RK2IMEX step; forced_dt NULL
+ * takes the CFL step 1/tstepdenmax. Synchronous; returns the first error
+ * any kernel of the step reported. */
+int kz_step(kz_sim *s, const double *forced_dt);
+
+/* Enqueue nsteps CFL steps on the sim's stream without waiting (the
+ * timestep stays on the device between steps); kz_sim_sync waits and
+ * returns the first device-side error. */
+int kz_step_many(kz_sim *s, int nsteps);
+int kz_sim_sync(kz_sim *s);
+/* The hipStream_t the sim enqueues on. */
+void *kz_sim_stream(kz_sim *s);
+
+/* The Sim fields the reference's self-goldens record (t, dt, own_dt,
+ * tstepdenmax/min, nstep, implicit-solve counters). Synchronous. */
+int kz_sim_counters(kz_sim *s, kz_counters *c);
+
+/* Kernel classes of a step, for event timing. */
+enum {
+  KZ_K_WAVESPEEDS = 0, /* calcWavespeeds */
+  KZ_K_FACE_FLUX = 1,  /* fused PPM + floors + fFluxPrime + LAXF, per dim */
+  KZ_K_CT = 2,         /* EMF + flux rebuild */
+  KZ_K_UPDATE = 3,     /* flux divergence + metric source */
+  KZ_K_U2P = 4,        /* u2p + floors */
+  KZ_K_IMPLICIT = 5,   /* implicit four-force Newton */
+  KZ_K_BC = 6,         /* boundary conditions */
+  KZ_K_COUNT = 7
+};
+/* Time every launch of one kernel class with HIP events recorded on the
+ * sim's stream (kernel = -1 turns timing off); kz_sim_kernel_time returns
+ * the summed elapsed milliseconds and the launch count since the last
+ * call and restarts the accumulation.  Profiling aid with no counterpart
+ * in the reference (its per-phase timers are host wall clocks). */
+int kz_sim_time_kernel(kz_sim *s, int kernel);
+int kz_sim_kernel_time(kz_sim *s, double *ms_total, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KZ_STEP_H */

@@ -50,9 +50,11 @@ KZM_QUAL double kzm_fabs(double x) { return __builtin_fabs(x); }
 KZM_QUAL double kzm_sqrt(double x) { return __builtin_sqrt(x); }
 KZM_QUAL int kzm_isnan(double x) { return x != x; }
 KZM_QUAL int kzm_isfinite(double x) { return kzm_fabs(x) <= 0x1.fffffffffffffp+1023; }
-/* Zig @min/@max semantics: a NaN operand yields the other operand. */
-KZM_QUAL double kzm_fmin(double a, double b) { return __builtin_fmin(a, b); }
-KZM_QUAL double kzm_fmax(double a, double b) { return __builtin_fmax(a, b); }
+/* @min/@max with a NaN operand yield the other operand (koral/math/simd.zig
+ * contract).  Written as explicit comparisons so that ties (+0/-0) resolve
+ * the same way on both compilers: an equal pair returns b. */
+KZM_QUAL double kzm_fmin(double a, double b) { return (a < b || b != b) ? a : b; }
+KZM_QUAL double kzm_fmax(double a, double b) { return (a > b || b != b) ? a : b; }
 
 #define KZM_INF (__builtin_inf())
 #define KZM_NAN (__builtin_nan(""))

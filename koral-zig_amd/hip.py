@@ -1,0 +1,318 @@
+"""Binding of libkz_hip.so — the per-step hot path on the GPU.
+
+`Sim` mirrors the reference's Sim(config.puffy) methods (koral/sim.zig:
+93-1900) over the C-ABI of include/kz_step.h; the *_batch functions expose
+the per-point device physics of include/kz_cellops.h.  There is no CPU
+fallback: every call runs the HIP kernels, and a missing library raises.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import abi_types as T
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libkz_hip.so"
+_lib = None
+
+D = C.POINTER(C.c_double)
+I32 = C.POINTER(C.c_int32)
+GEOM = C.POINTER(C.c_double)  # kz_geom records passed as packed doubles
+
+# every entry point the headers declare: name -> (restype, argtypes)
+_SIGS = {
+    # kz_step.h
+    "kz_sim_create": (C.c_void_p, [C.POINTER(T.kz_grid), C.POINTER(T.kz_physics)] + [D] * 10 + [I32]),
+    "kz_sim_destroy": (None, [C.c_void_p]),
+    "kz_sim_set_state": (C.c_int, [C.c_void_p, D, D]),
+    "kz_sim_get_state": (C.c_int, [C.c_void_p, D, D, I32]),
+    "kz_sim_p2u_domain": (C.c_int, [C.c_void_p]),
+    "kz_sim_set_bc": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_sim_postinit": (C.c_int, [C.c_void_p, C.c_double, D]),
+    "kz_sim_init_timestep_guess": (C.c_int, [C.c_void_p]),
+    "kz_sim_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "kz_calc_u2p": (C.c_int, [C.c_void_p]),
+    "kz_op_explicit": (C.c_int, [C.c_void_p, C.c_double]),
+    "kz_op_implicit": (C.c_int, [C.c_void_p, C.c_double]),
+    "kz_do_correct": (C.c_int, [C.c_void_p]),
+    "kz_step": (C.c_int, [C.c_void_p, D]),
+    "kz_step_many": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_sim_sync": (C.c_int, [C.c_void_p]),
+    "kz_sim_stream": (C.c_void_p, [C.c_void_p]),
+    "kz_sim_counters": (C.c_int, [C.c_void_p, C.POINTER(T.kz_counters)]),
+    "kz_sim_time_kernel": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_sim_kernel_time": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_uint64)]),
+    # kz_cellops.h
+    "kz_p2u_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, D]),
+    "kz_u2p_mhd_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.c_double, C.POINTER(T.kz_floor_params), C.c_void_p]),
+    "kz_check_floors_mhd_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, C.POINTER(T.kz_floor_params), I32,
+                                            C.POINTER(C.c_uint64)]),
+    "kz_u2p_rad_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.POINTER(T.kz_rad_params), C.c_void_p]),
+    "kz_check_floors_rad_batch": (C.c_int, [C.c_int64, D, GEOM, C.POINTER(T.kz_rad_params), I32]),
+    "kz_solve_implicit_lab_batch": (C.c_int, [C.c_int64, D, D, GEOM, D, C.c_double, C.POINTER(T.kz_rad_params),
+                                              C.POINTER(T.kz_opac_params), C.POINTER(T.kz_implicit_params),
+                                              C.c_void_p]),
+    "kz_ppm_batch": (C.c_int, [C.c_int64, D, D, D, D]),
+    "kz_flux_prime_batch": (C.c_int, [C.c_int64, D, I32, GEOM, C.c_double, D, I32]),
+    "kz_gas_wavespeeds_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, I32, D]),
+    "kz_rad_wavespeeds_batch": (C.c_int, [C.c_int64, D, GEOM, D, I32, D]),
+    "kz_rij_batch": (C.c_int, [C.c_int64, D, GEOM, D]),
+    "kz_radstate_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, C.POINTER(T.kz_opac_params), C.c_int32, D]),
+    "kz_math_batch": (C.c_int, [C.c_int64, C.c_int32, D, D, D]),
+}
+
+
+def lib():
+    """Load libkz_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: the HIP library must be built "
+                               "(__graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {T.ERRORS.get(rc, rc)}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def _geoms(g):
+    g = _f64(g)
+    assert g.shape[-1] == T.GEOM_DOUBLES, g.shape
+    return g
+
+
+class Sim:
+    """Device-resident run state driven through the kz_step.h C-ABI.
+
+    `tables` is the dict from koral_zig_amd.host.metric_tables(); it is
+    copied to the device at construction.
+    """
+
+    def __init__(self, grid, phys, tables):
+        self.grid = grid
+        self.phys = phys
+        err = C.c_int32(0)
+        t = {k: _f64(v) for k, v in tables.items() if isinstance(v, np.ndarray)}
+        self.h = lib().kz_sim_create(C.byref(grid), C.byref(phys), T.ptr(t["g"]), T.ptr(t["gcon"]),
+                                     T.ptr(t["kris"]), T.ptr(t["gb0"]), T.ptr(t["gconb0"]), T.ptr(t["gb1"]),
+                                     T.ptr(t["gconb1"]), T.ptr(t["gb2"]), T.ptr(t["gconb2"]), T.ptr(t["bcx"]),
+                                     C.byref(err))
+        if not self.h:
+            raise RuntimeError(f"kz_sim_create failed: {T.ERRORS.get(err.value, err.value)}")
+        (sz, sy, sx), _ = T.padded_shape(grid)
+        self.shape = (sz, sy, sx)
+        self.ncell = sz * sy * sx
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kz_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, p=None, u=None):
+        p = None if p is None else _f64(p)
+        u = None if u is None else _f64(u)
+        _chk(lib().kz_sim_set_state(self.h, T.ptr(p), T.ptr(u)), "kz_sim_set_state")
+
+    def get_state(self):
+        p = np.zeros(self.shape + (T.NV,))
+        u = np.zeros(self.shape + (T.NV,))
+        f = np.zeros(self.shape + (T.NFLAGS,), np.int32)
+        _chk(lib().kz_sim_get_state(self.h, T.ptr(p), T.ptr(u), T.ptr(f, C.c_int32)), "kz_sim_get_state")
+        return p, u, f
+
+    def counters(self):
+        c = T.kz_counters()
+        _chk(lib().kz_sim_counters(self.h, C.byref(c)), "kz_sim_counters")
+        return {n: getattr(c, n) for n, _ in T.kz_counters._fields_}
+
+    def set_time(self, t, tstepdenmax, tstepdenmin):
+        _chk(lib().kz_sim_set_time(self.h, t, tstepdenmax, tstepdenmin), "kz_sim_set_time")
+
+    def p2u_domain(self):
+        _chk(lib().kz_sim_p2u_domain(self.h), "kz_sim_p2u_domain")
+
+    def set_bc(self, ifinit=False):
+        _chk(lib().kz_sim_set_bc(self.h, int(ifinit)), "kz_sim_set_bc")
+
+    def calc_u2p(self):
+        _chk(lib().kz_calc_u2p(self.h), "kz_calc_u2p")
+
+    def op_explicit(self, dt):
+        _chk(lib().kz_op_explicit(self.h, dt), "kz_op_explicit")
+
+    def op_implicit(self, dt):
+        _chk(lib().kz_op_implicit(self.h, dt), "kz_op_implicit")
+
+    def do_correct(self):
+        _chk(lib().kz_do_correct(self.h), "kz_do_correct")
+
+    def init_timestep_guess(self):
+        _chk(lib().kz_sim_init_timestep_guess(self.h), "kz_sim_init_timestep_guess")
+
+    def postinit(self, maxbeta):
+        fac = C.c_double(0.0)
+        _chk(lib().kz_sim_postinit(self.h, maxbeta, C.byref(fac)), "kz_sim_postinit")
+        return fac.value
+
+    def step(self, dt=None):
+        if dt is None:
+            _chk(lib().kz_step(self.h, None), "kz_step")
+        else:
+            d = C.c_double(dt)
+            _chk(lib().kz_step(self.h, C.pointer(d)), "kz_step")
+
+    def step_many(self, nsteps):
+        """Enqueue nsteps CFL steps; call sync() to wait."""
+        _chk(lib().kz_step_many(self.h, int(nsteps)), "kz_step_many")
+
+    def sync(self):
+        _chk(lib().kz_sim_sync(self.h), "kz_sim_sync")
+
+    def stream(self):
+        return lib().kz_sim_stream(self.h)
+
+    def time_kernel(self, kernel):
+        """Start event-timing one kernel class (KERNELS name, or None to stop)."""
+        k = -1 if kernel is None else KERNELS[kernel]
+        _chk(lib().kz_sim_time_kernel(self.h, k), "kz_sim_time_kernel")
+
+    def kernel_time(self):
+        """(summed ms, launches) of the timed class since the last call."""
+        ms, n = C.c_double(0.0), C.c_uint64(0)
+        _chk(lib().kz_sim_kernel_time(self.h, C.byref(ms), C.byref(n)), "kz_sim_kernel_time")
+        return ms.value, n.value
+
+
+KERNELS = {"wavespeeds": 0, "face_flux": 1, "ct": 2, "update": 3, "u2p": 4, "implicit": 5, "bc": 6}
+
+
+# ---- per-point batches (include/kz_cellops.h) ------------------------------
+
+def p2u(pp, geom, gam):
+    pp = _f64(pp)
+    uu = np.zeros_like(pp)
+    _chk(lib().kz_p2u_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), gam, T.ptr(uu)), "kz_p2u_batch")
+    return uu
+
+
+def u2p_mhd(uu, pp, geom, gam, floors):
+    uu, pp = _f64(uu), _f64(pp).copy()
+    r = np.zeros(len(uu), T.U2P_RESULT_DTYPE)
+    _chk(lib().kz_u2p_mhd_batch(len(uu), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), gam, C.byref(floors),
+                                r.ctypes.data), "kz_u2p_mhd_batch")
+    return pp, r
+
+
+def check_floors_mhd(pp, geom, gam, floors):
+    pp = _f64(pp).copy()
+    ret = np.zeros(len(pp), np.int32)
+    ng = C.c_uint64(0)
+    _chk(lib().kz_check_floors_mhd_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), gam, C.byref(floors),
+                                         T.ptr(ret, C.c_int32), C.byref(ng)), "kz_check_floors_mhd_batch")
+    return pp, ret, ng.value
+
+
+def u2p_rad(uu, pp, geom, rad):
+    uu, pp = _f64(uu), _f64(pp).copy()
+    r = np.zeros(len(uu), T.U2PRAD_RESULT_DTYPE)
+    _chk(lib().kz_u2p_rad_batch(len(uu), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), C.byref(rad), r.ctypes.data),
+         "kz_u2p_rad_batch")
+    return pp, r
+
+
+def check_floors_rad(pp, geom, rad):
+    pp = _f64(pp).copy()
+    ret = np.zeros(len(pp), np.int32)
+    _chk(lib().kz_check_floors_rad_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), C.byref(rad),
+                                         T.ptr(ret, C.c_int32)), "kz_check_floors_rad_batch")
+    return pp, ret
+
+
+def solve_implicit(uu, pp, geom, dt, gam, rad, opac, imp):
+    uu, pp = _f64(uu).copy(), _f64(pp).copy()
+    n = len(uu)
+    dt = _f64(np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+    r = np.zeros(n, T.IMPLICIT_RESULT_DTYPE)
+    _chk(lib().kz_solve_implicit_lab_batch(n, T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(dt), gam,
+                                           C.byref(rad), C.byref(opac), C.byref(imp), r.ctypes.data),
+         "kz_solve_implicit_lab_batch")
+    return uu, pp, r
+
+
+def ppm(u5, dx5):
+    u5, dx5 = _f64(u5), _f64(dx5)
+    n = len(u5)
+    l, r = np.zeros(n), np.zeros(n)
+    _chk(lib().kz_ppm_batch(n, T.ptr(u5), T.ptr(dx5), T.ptr(l), T.ptr(r)), "kz_ppm_batch")
+    return l, r
+
+
+def flux_prime(pp, idim, geom, gam):
+    pp = _f64(pp)
+    n = len(pp)
+    idim = np.ascontiguousarray(np.broadcast_to(np.asarray(idim, np.int32), (n,)))
+    ff = np.zeros_like(pp)
+    err = np.zeros(n, np.int32)
+    _chk(lib().kz_flux_prime_batch(n, T.ptr(pp), T.ptr(idim, C.c_int32), T.ptr(_geoms(geom)), gam, T.ptr(ff),
+                                   T.ptr(err, C.c_int32)), "kz_flux_prime_batch")
+    return ff, err
+
+
+def gas_wavespeeds(pp, geom, gam, active=(1, 1, 1)):
+    pp = _f64(pp)
+    act = np.asarray(active, np.int32)
+    out = np.zeros((len(pp), 6))
+    _chk(lib().kz_gas_wavespeeds_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), gam, T.ptr(act, C.c_int32),
+                                       T.ptr(out)), "kz_gas_wavespeeds_batch")
+    return out
+
+
+def rad_wavespeeds(pp, geom, tau3, active=(1, 1, 1)):
+    pp, tau3 = _f64(pp), _f64(tau3)
+    act = np.asarray(active, np.int32)
+    out = np.zeros((len(pp), 12))
+    _chk(lib().kz_rad_wavespeeds_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(tau3),
+                                       T.ptr(act, C.c_int32), T.ptr(out)), "kz_rad_wavespeeds_batch")
+    return out
+
+
+def rij(pp, geom):
+    pp = _f64(pp)
+    out = np.zeros((len(pp), 21))
+    _chk(lib().kz_rij_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(out)), "kz_rij_batch")
+    return out
+
+
+def radstate(pp, geom, gam, opac, full=True):
+    pp = _f64(pp)
+    out = np.zeros((len(pp), 10))
+    _chk(lib().kz_radstate_batch(len(pp), T.ptr(pp), T.ptr(_geoms(geom)), gam, C.byref(opac), int(full),
+                                 T.ptr(out)), "kz_radstate_batch")
+    return out
+
+
+MATH_FN = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "log1p": 4, "sqrt": 5, "div": 6}
+
+
+def math_fn(name, x, y=None):
+    x = _f64(x)
+    yy = None if y is None else _f64(np.broadcast_to(np.asarray(y, np.float64), x.shape))
+    out = np.zeros_like(x)
+    _chk(lib().kz_math_batch(x.size, MATH_FN[name], T.ptr(x), T.ptr(yy), T.ptr(out)), "kz_math_batch")
+    return out

@@ -972,9 +972,13 @@ int kz_host_puffy_prepinit(const kz_grid *gr, double a, double mksr0, double mks
   double uintatmmin = (op->kb_over_mugas_mp * rhoatmmin * atm_tgas) / (gam - 1.0);
   double t3 = atm_trad_init;
   double eradatmmin = op->four_sigmarad * t3 * t3 * t3 * t3 / 10.0 * 6.62 / mass;
-  for (long iz = 0; iz < gr->nz; ++iz)
-    for (long iy = 0; iy < gr->ny; ++iy)
-      for (long ix = 0; ix < gr->nx; ++ix) {
+  // cells are independent (each runs its own torus quadrature): rows in parallel
+  const long nrows = (long)gr->nz * gr->ny;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long row = 0; row < nrows; ++row)
+    for (long ix = 0; ix < gr->nx; ++ix) {
+      const long iz = row / gr->ny, iy = row % gr->ny;
+      {
         long ci = G.cidx(ix, iy, iz);
         kz_geom geom;
         memset(&geom, 0, sizeof geom);
@@ -1046,6 +1050,7 @@ int kz_host_puffy_prepinit(const kz_grid *gr, double a, double mksr0, double mks
         }
         memcpy(p + ci * KZ_NV, pp, sizeof pp);
       }
+    }
   return 0;
 }
 
