@@ -45,9 +45,16 @@ int kz_solve_implicit_lab_batch(int64_t n, double *uu, double *pp, const kz_geom
                                 const kz_rad_params *rp, const kz_opac_params *op, const kz_implicit_params *ip,
                                 kz_implicit_result *r);
 
-/* reconstruct(.ppm) (koral/fv/recon.zig:47-56, 99-195): 5-point stencil
- * values and cell sizes -> left/right face values of the centre cell. */
-int kz_ppm_batch(int64_t n, const double *u5, const double *dx5, double *l, double *r);
+/* reconstruct (koral/fv/recon.zig:47-53): order 0 donor cell, 1 linear
+ * minmod with theta (:77-91), 2 PPM (:99-195); 5-point stencil values and
+ * cell sizes -> left/right face values of the centre cell. */
+int kz_reconstruct_batch(int64_t n, const int32_t *order, const double *theta, const double *u5, const double *dx5,
+                         double *l, double *r);
+
+/* u2pSolverW (koral/solve/invert.zig:235-404): the W Newton alone,
+ * etype 0 hot (energy) / 1 entropy; pp in = guess, out = result. */
+int kz_u2p_solver_w_batch(int64_t n, const double *uu, double *pp, const kz_geom *g, int32_t etype, double gam,
+                          int32_t *ret);
 
 /* fFluxPrime (koral/physics/flux.zig:40-134); err[i] = KZ_E_NAN_IN_FLUX
  * when a component is not finite. idim in 0..2. */

@@ -129,7 +129,8 @@ __global__ void __launch_bounds__(64) b_implicit(long n, double *uu, double *pp,
   }
 }
 
-__global__ void b_ppm(long n, const double *u5, const double *dx5, double *l, double *r) {
+__global__ void b_reconstruct(long n, const int32_t *order, const double *theta, const double *u5, const double *dx5,
+                              double *l, double *r) {
   long i = TID;
   if (i >= n) return;
   double u[5], d[5];
@@ -137,7 +138,22 @@ __global__ void b_ppm(long n, const double *u5, const double *dx5, double *l, do
     u[k] = u5[i * 5 + k];
     d[k] = dx5[i * 5 + k];
   }
-  ppm(u, d, l + i, r + i);
+  reconstruct(order[i], theta[i], u, d, l + i, r + i);
+}
+
+__global__ void b_u2p_solver_w(long n, const double *uu, double *pp, const kz_geom *g, int etype, double gam,
+                               int32_t *ret) {
+  long i = TID;
+  if (i >= n) return;
+  Geo gm;
+  geo_from_kz(g[i], gm);
+  double p[NV], u[NV];
+  for (int k = 0; k < NV; ++k) {
+    p[k] = pp[i * NV + k];
+    u[k] = uu[i * NV + k];
+  }
+  ret[i] = u2p_solver_w(u, p, gm, etype, gam);
+  for (int k = 0; k < NV; ++k) pp[i * NV + k] = p[k];
 }
 
 __global__ void b_flux_prime(long n, const double *pp, const int32_t *idim, const kz_geom *g, double gam, double *ff,
@@ -370,15 +386,34 @@ int kz_solve_implicit_lab_batch(int64_t n, double *uu, double *pp, const kz_geom
   return s.bad ? KZ_E_HIP : 0;
 }
 
-int kz_ppm_batch(int64_t n, const double *u5, const double *dx5, double *l, double *r) {
+int kz_reconstruct_batch(int64_t n, const int32_t *order, const double *theta, const double *u5, const double *dx5,
+                         double *l, double *r) {
   if (n <= 0) return 0;
   Stage s;
-  double *du = s.in(u5, n * 5), *dd = s.in(dx5, n * 5), *dl = s.alloc<double>(n), *dr = s.alloc<double>(n);
+  int32_t *dor = s.in(order, n);
+  double *dth = s.in(theta, n), *du = s.in(u5, n * 5), *dd = s.in(dx5, n * 5);
+  double *dl = s.alloc<double>(n), *dr = s.alloc<double>(n);
   if (s.bad) return KZ_E_HIP;
-  b_ppm<<<nb(n, 256), 256>>>(n, du, dd, dl, dr);
+  b_reconstruct<<<nb(n, 256), 256>>>(n, dor, dth, du, dd, dl, dr);
   if (int e = s.finish_launch()) return e;
   s.out(l, dl, n);
   s.out(r, dr, n);
+  return s.bad ? KZ_E_HIP : 0;
+}
+
+int kz_u2p_solver_w_batch(int64_t n, const double *uu, double *pp, const kz_geom *g, int32_t etype, double gam,
+                          int32_t *ret) {
+  if (n <= 0) return 0;
+  if (etype != 0 && etype != 1) return KZ_E_INVALID_CONFIG;
+  Stage s;
+  double *du = s.in(uu, n * NV), *dp = s.in(pp, n * NV);
+  kz_geom *dg = s.in(g, n);
+  int32_t *dr = s.alloc<int32_t>(n);
+  if (s.bad) return KZ_E_HIP;
+  b_u2p_solver_w<<<nb(n, 64), 64>>>(n, du, dp, dg, etype, gam, dr);
+  if (int e = s.finish_launch()) return e;
+  s.out(pp, dp, n * NV);
+  s.out(ret, dr, n);
   return s.bad ? KZ_E_HIP : 0;
 }
 

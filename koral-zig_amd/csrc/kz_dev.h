@@ -521,6 +521,27 @@ KZD void ppm(const double u[5], const double dx[5], double *lo, double *ro) {
   *lo = left;
   *ro = right;
 }
+// recon.zig:77-91 linear reconstruction with the generalized minmod limiter
+KZD void minmod_linear(const double u[5], double theta, double *lo, double *ro) {
+  double dm = u[2] - u[1], dp = u[3] - u[2];
+  double avg = 0.5 * (dm + dp);
+  double lo_s = kzm_fmin(kzm_fmin(theta * dm, avg), theta * dp);
+  double hi_s = kzm_fmax(kzm_fmax(theta * dm, avg), theta * dp);
+  double slope = (dm * dp <= 0.0) ? 0.0 : (dm > 0.0 ? lo_s : hi_s);
+  *lo = u[2] - 0.5 * slope;
+  *ro = u[2] + 0.5 * slope;
+}
+// recon.zig:47-53 reconstruct: order 0 donor cell, 1 linear minmod, 2 PPM
+KZD void reconstruct(int order, double theta, const double u[5], const double dx[5], double *lo, double *ro) {
+  if (order <= 0) {
+    *lo = u[2];
+    *ro = u[2];
+  } else if (order == 1) {
+    minmod_linear(u, theta, lo, ro);
+  } else {
+    ppm(u, dx, lo, ro);
+  }
+}
 KZD double laxf(double fl, double fr, double ul, double ur, double ag) { return 0.5 * (fr + fl - ag * (ur - ul)); }
 
 // ---- physics/flux.zig:40-134 fFluxPrime -------------------------------------

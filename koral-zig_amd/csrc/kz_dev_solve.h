@@ -536,11 +536,14 @@ struct ImpCtx {
   double opdamp;
 };
 
-// implicit.zig:240-306 applyConstraints
-KZD int apply_constraints(double *pp, double *uu, const double *uu0, const ImpCtx &cx, int whichprim) {
+// implicit.zig:240-306 applyConstraints.  WP (the primitive block the
+// Newton solves for) is a template parameter so that every state index is
+// a compile-time constant and the 13-slot states stay in registers.
+template <int WP>
+KZD int apply_constraints(double *pp, double *uu, const double *uu0, const ImpCtx &cx) {
   const Geo &gm = *cx.gm;
   double gdetu = gm.gdet, gi = 1.0 / gdetu;
-  if (whichprim == WP_MHD) {
+  if (WP == WP_MHD) {
     double ucon[4], ucov[4];
     ucon_ucov(pp, gm, ucon, ucov);
     pp[RHO] = uu0[RHO] * gi / ucon[0];
@@ -573,18 +576,16 @@ KZD int apply_constraints(double *pp, double *uu, const double *uu0, const ImpCt
 struct State0 {
   double ehat, uint, sgas;
 };
+// the reference-state opacities lagOpac substitutes (implicit.zig:631-640)
+struct LagOpac {
+  double kappaes;
+  Opac op;
+};
 
-KZD void fill_state(const double *pp, const ImpCtx &cx, bool slim, const RadState &st00, RadState &st) {
-  fill_radstate(pp, *cx.gm, cx.gam, *cx.op, !slim, st);
-  if (cx.ip->lag_opac) {  // implicit.zig:631-640
-    st.kappaes = st00.kappaes;
-    st.op = st00.op;
-  }
-}
-
-// implicit.zig:337-427 residualG (scalar)
+// implicit.zig:337-427 residualG (scalar) on a filled state
+template <int WP>
 KZD double residual(const double *uu, const double *pp, const RadState &st, const double *uu0, const State0 &s0,
-                    double dt, const ImpCtx &cx, int wp, int we, int wf, double f[4]) {
+                    double dt, const ImpCtx &cx, int we, int wf, double f[4]) {
   const Geo &gm = *cx.gm;
   double gdetu = gm.gdet;
   double dtau = dt / st.ucon[0];
@@ -593,20 +594,18 @@ KZD double residual(const double *uu, const double *pp, const RadState &st, cons
   lower4(gilab, gm, gi);
   double err[4] = {0, 0, 0, 0};
   const double SMALLI = 1.0e-80;
-  int base = (wp == WP_MHD) ? UU : EE;
-  double sgn_ = (wp == WP_MHD) ? -1.0 : 1.0;
+  constexpr int base = (WP == WP_MHD) ? UU : EE;
 #pragma unroll
   for (int k = 1; k < 4; ++k) {
     // MHD: uu - uu0 - dt*gdet*G;  RAD: uu - uu0 + dt*gdet*G
-    f[k] = (wp == WP_MHD) ? uu[base + k] - uu0[base + k] - dt * gdetu * gi[k]
+    f[k] = (WP == WP_MHD) ? uu[base + k] - uu0[base + k] - dt * gdetu * gi[k]
                           : uu[base + k] - uu0[base + k] + dt * gdetu * gi[k];
     err[k] = (kzm_fabs(f[k]) > SMALLI) ? kzm_fabs(f[k]) / (1.0e-20 * uu[base] + kzm_fabs(uu[base + k]) +
                                                            kzm_fabs(uu0[base + k]) + kzm_fabs(dt * gdetu * gi[k]))
                                        : 0.0;
   }
-  (void)sgn_;
   if (wf == WF_LAB) {
-    if (wp == WP_RAD) {
+    if (WP == WP_RAD) {
       f[0] = uu[EE] - uu0[EE] + dt * gdetu * gi[0];
       err[0] = (kzm_fabs(f[0]) > SMALLI)
                    ? kzm_fabs(f[0]) / (kzm_fabs(uu[EE]) + kzm_fabs(uu0[EE]) + kzm_fabs(dt * gdetu * gi[0]))
@@ -618,7 +617,7 @@ KZD double residual(const double *uu, const double *pp, const RadState &st, cons
                    : 0.0;
     }
   } else if (we == WE_ENERGY) {
-    if (wp == WP_RAD) {
+    if (WP == WP_RAD) {
       f[0] = st.ehat - s0.ehat + dtau * giff0;
       err[0] = kzm_fabs(f[0]) / (kzm_fabs(st.ehat) + kzm_fabs(s0.ehat) + kzm_fabs(dtau * giff0));
     } else {
@@ -633,6 +632,23 @@ KZD double residual(const double *uu, const double *pp, const RadState &st, cons
 #pragma unroll
   for (int k = 0; k < 4; ++k) e0 = err[k] > e0 ? err[k] : e0;
   return e0;
+}
+
+// The residual at pp: fill the radiation state (slim for the energy
+// equations, full for entropy), apply lagOpac, evaluate.  The reference
+// fills the state right after each applyConstraints and consumes it only
+// here, with pp unchanged in between, so filling at the point of use gives
+// the same values and keeps the state a temporary.
+template <int WP>
+KZD double eval_residual(const double *uu, const double *pp, const LagOpac &lag, const double *uu0,
+                         const State0 &s0, double dt, const ImpCtx &cx, int we, int wf, double f[4]) {
+  RadState st;
+  fill_radstate(pp, *cx.gm, cx.gam, *cx.op, we == WE_ENTROPY, st);
+  if (cx.ip->lag_opac) {
+    st.kappaes = lag.kappaes;
+    st.op = lag.op;
+  }
+  return residual<WP>(uu, pp, st, uu0, s0, dt, cx, we, wf, f);
 }
 
 // implicit.zig:431-467 f1dErr (energy equation, undamped opacities)
@@ -659,15 +675,17 @@ KZD double f1d_err(double xxx, const double *pp0, double dtau, const ImpCtx &cx,
   return ugas - pp0[UU] - dtau * giff0;
 }
 
-// implicit.zig:473-574 solve1dPrim
-KZD void solve_1d_prim(const double *pp00, const RadState &st00, double dt, const ImpCtx &cx, double *pp0out) {
+// implicit.zig:473-574 solve1dPrim: bisection warm start on the fluid-frame
+// energy equation; reads Ehat, u^t and T_gas of the reference state.
+KZD void solve_1d_prim(const double *pp00, double st_ehat, double st_ut, double st_tgas, double dt, const ImpCtx &cx,
+                       double *pp0out) {
 #pragma unroll
   for (int i = 0; i < NV; ++i) pp0out[i] = pp00[i];
-  double ehat = st00.ehat;
-  double dtau = dt / st00.ucon[0];
+  double ehat = st_ehat;
+  double dtau = dt / st_ut;
   double tot = ehat + pp00[UU];
   int wp = (ehat < 1.0e-3 * pp00[UU]) ? WP_RAD : WP_MHD;
-  double tgas = st00.tgas;
+  double tgas = st_tgas;
   double ratio = ehat / pp00[EE];
   const double fac = 3.0, fac1 = fac + 1.0;
   double pu = pp00[UU], pe = pp00[EE];
@@ -728,15 +746,29 @@ KZD void solve_1d_prim(const double *pp00, const RadState &st00, double dt, cons
   }
 }
 
-KZD double fd_del(int j, double sign, const double *ppp, int sh, double eps, const Geo &gm) {  // :586-591
-  if (j == 0) return sign * eps * ppp[sh];
-  double veleps = eps * my_sign(ppp[j + sh]) * kzm_fmax(1.0e-8 / kzm_sqrt(gm.g[j][j]), kzm_fabs(ppp[j + sh]));
-  return ppp[j + sh] >= 0.0 ? sign * veleps : -sign * veleps;
+// Element j (0..3, not a compile-time constant) of a 4-block, by selects,
+// so that register-resident arrays are never indexed dynamically.
+KZD double get4(const double *a, int j) { return j == 0 ? a[0] : (j == 1 ? a[1] : (j == 2 ? a[2] : a[3])); }
+KZD void set4(double *a, int j, double v) {
+  if (j == 0) a[0] = v;
+  else if (j == 1) a[1] = v;
+  else if (j == 2) a[2] = v;
+  else a[3] = v;
+}
+KZD double gdiag(const Geo &gm, int j) { return j == 1 ? gm.g[1][1] : (j == 2 ? gm.g[2][2] : gm.g[3][3]); }
+
+// implicit.zig:586-591 fdDel on the block b = ppp + sh
+KZD double fd_del(int j, double sign, const double *b, double eps, const Geo &gm) {
+  if (j == 0) return sign * eps * b[0];
+  double bj = get4(b, j);
+  double veleps = eps * my_sign(bj) * kzm_fmax(1.0e-8 / kzm_sqrt(gdiag(gm, j)), kzm_fabs(bj));
+  return bj >= 0.0 ? sign * veleps : -sign * veleps;
 }
 
 // implicit.zig:642-922 solve4dPrimCore (scalar Jacobian); returns ok
-KZD bool solve_4d_prim_core(const double *uu00, const RadState &st00, const State0 &s0, const double *pp0in,
-                            double dt, const ImpCtx &cx, int wp, int we, int wf, double *pp_out, int *iters_out) {
+template <int WP>
+__device__ static __forceinline__ bool solve_4d_prim_core(const double *uu00, const LagOpac &lag, const State0 &s0, const double *pp0in, double dt,
+                            const ImpCtx &cx, int we, int wf, double *pp_out, int *iters_out) {
   const kz_implicit_params &ip = *cx.ip;
   const Geo &gm = *cx.gm;
   double uu[NV], pp[NV], ppp[NV];
@@ -745,12 +777,9 @@ KZD bool solve_4d_prim_core(const double *uu00, const RadState &st00, const Stat
     uu[i] = uu00[i];
     pp[i] = pp0in[i];
   }
-  const int sh = (wp == WP_MHD) ? UU : EE;
+  constexpr int sh = (WP == WP_MHD) ? UU : EE;
   const double conv = (we == WE_ENTROPY) ? ip.entrconv : ip.conv;
-  int fret = apply_constraints(pp, uu, uu00, cx, wp);
-  const bool slim = we != WE_ENTROPY;
-  RadState state;
-  fill_state(pp, cx, slim, st00, state);
+  int fret = apply_constraints<WP>(pp, uu, uu00, cx);
   double f1[4], f2[4], jac[4][4];
   int iter = 0;
   bool failed = false;
@@ -759,28 +788,28 @@ KZD bool solve_4d_prim_core(const double *uu00, const RadState &st00, const Stat
 #pragma unroll
     for (int i = 0; i < NV; ++i) ppp[i] = pp[i];
     if (fret < -1) return false;
-    double errbase = residual(uu, pp, state, uu00, s0, dt, cx, wp, we, wf, f1);
+    double errbase = eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f1);
     if (errbase < conv) break;
+#pragma unroll 1
     for (int j = 0; j < 4; ++j) {
       double sign = -1.0;
       for (;;) {
-        double del = fd_del(j, sign, ppp, sh, ip.eps, gm);
-        pp[j + sh] = ppp[j + sh] + del;
-        fret = apply_constraints(pp, uu, uu00, cx, wp);
-        fill_state(pp, cx, slim, st00, state);
+        double del = fd_del(j, sign, ppp + sh, ip.eps, gm);
+        set4(pp + sh, j, get4(ppp + sh, j) + del);
+        fret = apply_constraints<WP>(pp, uu, uu00, cx);
         if (fret < -1) {
           if (sign > 0.0) {
             failed = true;
             break;
           }
-          pp[j + sh] = ppp[j + sh];
+          set4(pp + sh, j, get4(ppp + sh, j));
           sign *= -1.0;
           continue;
         }
-        residual(uu, pp, state, uu00, s0, dt, cx, wp, we, wf, f2);
+        eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f2);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) jac[i][j] = (f2[i] - f1[i]) / del;
-        pp[j + sh] = ppp[j + sh];
+        for (int i = 0; i < 4; ++i) set4(jac[i], j, (f2[i] - f1[i]) / del);
+        set4(pp + sh, j, get4(ppp + sh, j));
         break;
       }
       if (failed) break;
@@ -815,8 +844,7 @@ KZD bool solve_4d_prim_core(const double *uu00, const RadState &st00, const Stat
         }
 #pragma unroll
       for (int k = 0; k < 4; ++k) pp[k + sh] = xxx[k];
-      fret = apply_constraints(pp, uu, uu00, cx, wp);
-      fill_state(pp, cx, slim, st00, state);
+      fret = apply_constraints<WP>(pp, uu, uu00, cx);
       bool ok = false;
       double edenmin = ppp[sh] / ip.max_en_change_down;
       double edenmax = ppp[sh] * ip.max_en_change_up;
@@ -875,15 +903,21 @@ KZD kz_implicit_result solve_implicit_lab(double *uu, double *pp, const Geo &gm,
   for (int i = 0; i < NV; ++i) pp00[i] = pp[i];
   p2u(pp00, uu00, gm, gam);
   double ehat = -calc_ff_rtt(pp00, gm);
-  RadState st00;
-  fill_radstate(pp00, gm, gam, op, true, st00);
-  State0 s0{st00.ehat, st00.uint, st00.sgas};
+  State0 s0;
+  LagOpac lag;
   double pp0[NV];
-  if (ip.start_with_bisect) {
-    solve_1d_prim(pp00, st00, dt, cx, pp0);
-  } else {
+  {
+    RadState st00;
+    fill_radstate(pp00, gm, gam, op, true, st00);
+    s0 = State0{st00.ehat, st00.uint, st00.sgas};
+    lag.kappaes = st00.kappaes;
+    lag.op = st00.op;
+    if (ip.start_with_bisect) {
+      solve_1d_prim(pp00, st00.ehat, st00.ucon[0], st00.tgas, dt, cx, pp0);
+    } else {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
+      for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
+    }
   }
   const int sw = (ehat < ip.threshold * pp00[UU]) ? WP_RAD : WP_MHD;
   const int ot = (sw == WP_RAD) ? WP_MHD : WP_RAD;
@@ -897,7 +931,9 @@ KZD kz_implicit_result solve_implicit_lab(double *uu, double *pp, const Geo &gm,
       int wf = (ir == 0 || ir == 2) ? WF_LAB : WF_FF;
       double ppw[NV];
       int iters = 0;
-      if (solve_4d_prim_core(uu00, st00, s0, pp0, dt, cx, wp, we, wf, ppw, &iters)) {
+      bool ok = (wp == WP_MHD) ? solve_4d_prim_core<WP_MHD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters)
+                               : solve_4d_prim_core<WP_RAD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters);
+      if (ok) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) pp[i] = ppw[i];
         p2u(ppw, uu, gm, gam);

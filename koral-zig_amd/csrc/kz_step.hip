@@ -54,7 +54,15 @@ struct Dev {
   int *err;
   double *ss;  // step scalars (see k_begin_step)
   double *stage;  // staging for AoS <-> SoA
+  // split implicit solve: per domain cell (index t = ix + nx*(iy + ny*iz))
+  double *imp_uu00;  // [13][nd] conserved state of the incoming primitives
+  double *imp_pp0;   // [13][nd] Newton start (after the bisection warm start)
+  double *imp_aux;   // [IMP_NAUX][nd] reference-state scalars, see ImpAux
+  int32_t *imp_slow;           // cells that need rungs past the first
+  unsigned long long *imp_nslow;
 };
+// imp_aux planes
+enum { IA_EHAT = 0, IA_UINT, IA_SGAS, IA_KES, IA_KGA, IA_KRA, IA_KRR, IA_SW, IMP_NAUX };
 
 enum { SC_AHD = 0, SC_ARAD = 3, SC_TSTEPDEN = 6, SC_N = 7 };
 enum { SS_DT = 0, SS_DTG, SS_INV_DTG, SS_NINV_DTG, SS_INV_DT, SS_NINV_DT, SS_C12, SS_HALF, SS_T, SS_OWN_DT, SS_USER, SS_N };
@@ -174,11 +182,8 @@ __device__ void geo_from_rec(const double *rec, Geo &o) {
   o.gttpert = 0;
   o.x1 = 0;
 }
-// puffy.zig:1089-1166 (xhi) / :1167-1171 (xlo)
-__global__ void k_bc_x(GridD g, Dev d, kz_physics ph) {
-  long t = TID;
-  long n = (long)g.nz * g.ny * g.ngx * 2;
-  if (t >= n) return;
+// puffy.zig:1089-1166 (xhi) / :1167-1171 (xlo); t < nz*ny*ngx*2
+__device__ void bc_x_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
   int side = t % 2;
   long i = (t / 2) % g.ngx + 1;
   long iy = (t / (2 * g.ngx)) % g.ny;
@@ -231,11 +236,8 @@ __global__ void k_bc_x(GridD g, Dev d, kz_physics ph) {
   st(d.p, g.ncell, c, pp);
   st(d.u, g.ncell, c, uu);
 }
-// puffy.zig:1172-1187 polar reflection
-__global__ void k_bc_y(GridD g, Dev d, kz_physics ph) {
-  long t = TID;
-  long n = (long)g.nz * g.nx * g.ngy * 2;
-  if (t >= n) return;
+// puffy.zig:1172-1187 polar reflection; t < nz*nx*ngy*2
+__device__ void bc_y_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
   int side = t % 2;
   long i = (t / 2) % g.ngy + 1;
   long ix = (t / (2 * g.ngy)) % g.nx;
@@ -255,8 +257,24 @@ __global__ void k_bc_y(GridD g, Dev d, kz_physics ph) {
   st(d.p, g.ncell, c, pp);
   st(d.u, g.ncell, c, uu);
 }
-// bc.zig:286-377 fillCorners2d: one thread per corner quadrant (quadrants
-// never read each other's cells), each running its copies/averages in order.
+// x and y ghost cells in one launch: the x pass reads only domain cells of
+// rows 0..ny-1 and writes x ghosts, the y pass reads only domain cells and
+// writes y ghosts, so the two are independent (bc.zig:40-73 runs them in
+// sequence).
+__global__ void __launch_bounds__(64) k_bc_xy(GridD g, Dev d, kz_physics ph) {
+  long t = TID;
+  long nxg = (long)g.nz * g.ny * g.ngx * 2;
+  long nyg = (long)g.nz * g.nx * g.ngy * 2;
+  if (t < nxg) bc_x_cell(g, d, ph, t);
+  else if (t < nxg + nyg) bc_y_cell(g, d, ph, t - nxg);
+}
+
+// bc.zig:286-377 fillCorners2d.  Per corner quadrant the reference copies
+// 2(ng-1) cells from x/y ghosts, averages the diagonal cell next to the
+// domain from an x and a y ghost, then averages the outer diagonal cell from
+// two of the copied cells.  The first group reads no cell of its own group,
+// so it runs one thread per cell; the outer diagonals follow a barrier.
+// Quadrants never read each other's cells.  One block of 64 threads.
 __device__ void cp_p2u(const GridD &g, const Dev &d, double gam, long dx_, long dy_, long sx_, long sy_) {
   double pp[NV], uu[NV];
   ld(d.p, g.ncell, cidx(g, sx_, sy_, 0), pp);
@@ -281,44 +299,43 @@ __device__ void avg_p2u(const GridD &g, const Dev &d, double gam, long dx_, long
   st(d.p, g.ncell, c, pp);
   st(d.u, g.ncell, c, uu);
 }
-__global__ void k_bc_corners2d(GridD g, Dev d, kz_physics ph) {
-  int q = threadIdx.x;
-  long nx = g.nx, ny = g.ny, ng = g.ng;
-  double gam = ph.gam;
-  if (q == 0) {
-    for (long i = 0; i < ng - 1; ++i) {
-      cp_p2u(g, d, gam, -ng + i, -1, -ng + i, 0);
-      cp_p2u(g, d, gam, -1, -ng + i, 0, -ng + i);
+__global__ void __launch_bounds__(64) k_bc_corners2d(GridD g, Dev d, kz_physics ph) {
+  const long nx = g.nx, ny = g.ny, ng = g.ng;
+  const double gam = ph.gam;
+  const int per = 2 * (g.ng - 1) + 1;  // copies + inner diagonal per quadrant
+  const int tid = threadIdx.x;
+  if (tid < 4 * per) {
+    const int q = tid / per, k = tid % per;
+    // quadrant q: bit 0 = high y side, bit 1 = high x side
+    const bool hx = q & 2, hy = q & 1;
+    const long xin = hx ? nx - 1 : 0, yin = hy ? ny - 1 : 0;    // edge domain index
+    const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;          // first ghost index
+    if (k < 2 * (g.ng - 1)) {
+      const long i = k / 2;
+      if (k % 2 == 0) {
+        // ghost column beyond x, first ghost row in y: copy from the edge row
+        const long x = hx ? nx + i + 1 : -ng + i;
+        cp_p2u(g, d, gam, x, yg1, x, yin);
+      } else {
+        const long y = hy ? ny + i + 1 : -ng + i;
+        cp_p2u(g, d, gam, xg1, y, xin, y);
+      }
+    } else {
+      avg_p2u(g, d, gam, xg1, yg1, xg1, yin, xin, yg1);
     }
-    avg_p2u(g, d, gam, -1, -1, -1, 0, 0, -1);
-    avg_p2u(g, d, gam, -2, -2, -2, -1, -1, -2);
-  } else if (q == 1) {
-    for (long i = 0; i < ng - 1; ++i) {
-      cp_p2u(g, d, gam, -ng + i, ny, -ng + i, ny - 1);
-      cp_p2u(g, d, gam, -1, ny + i + 1, 0, ny + i + 1);
-    }
-    avg_p2u(g, d, gam, -1, ny, -1, ny - 1, 0, ny);
-    avg_p2u(g, d, gam, -2, ny + 1, -2, ny, -1, ny + 1);
-  } else if (q == 2) {
-    for (long i = 0; i < ng - 1; ++i) {
-      cp_p2u(g, d, gam, nx + i + 1, -1, nx + i + 1, 0);
-      cp_p2u(g, d, gam, nx, -ng + i, nx - 1, -ng + i);
-    }
-    avg_p2u(g, d, gam, nx, -1, nx - 1, -1, nx, 0);
-    avg_p2u(g, d, gam, nx + 1, -2, nx, -2, nx + 1, -1);
-  } else if (q == 3) {
-    for (long i = 0; i < ng - 1; ++i) {
-      cp_p2u(g, d, gam, nx + i + 1, ny, nx + i + 1, ny - 1);
-      cp_p2u(g, d, gam, nx, ny + i + 1, nx - 1, ny + i + 1);
-    }
-    avg_p2u(g, d, gam, nx, ny, nx - 1, ny, nx, ny - 1);
-    avg_p2u(g, d, gam, nx + 1, ny + 1, nx, ny + 1, nx + 1, ny);
+  }
+  __syncthreads();
+  if (tid < 4) {
+    const bool hx = tid & 2, hy = tid & 1;
+    const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;
+    const long xg2 = hx ? nx + 1 : -2, yg2 = hy ? ny + 1 : -2;
+    avg_p2u(g, d, gam, xg2, yg2, xg2, yg1, xg1, yg2);
   }
 }
 
 // ---------------------------------------------------------------------------
 // u2p pass (sim.zig:940-1000)
-__global__ void k_u2p(GridD g, Dev d, kz_physics ph) {
+__global__ void __launch_bounds__(128) k_u2p(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   if (t >= (long)g.nx * g.ny * g.nz) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
@@ -357,7 +374,7 @@ __global__ void k_u2p(GridD g, Dev d, kz_physics ph) {
 // cell_fixup (sim.zig:1027-1133), in place: a flagged cell reads only
 // unflagged neighbours, which no thread writes, so the reference's
 // copy-to-backup staging is unnecessary.
-__global__ void k_fixup(GridD g, Dev d, kz_physics ph, int which) {
+__global__ void __launch_bounds__(256) k_fixup(GridD g, Dev d, kz_physics ph, int which) {
   long t = TID;
   if (t >= (long)g.nx * g.ny * g.nz) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
@@ -401,7 +418,7 @@ __global__ void k_fixup(GridD g, Dev d, kz_physics ph, int which) {
 // ---------------------------------------------------------------------------
 // wavespeeds + CFL denominators (sim.zig:747-886)
 __device__ __forceinline__ long long dbits(double x) { return __double_as_longlong(x); }
-__global__ void k_wavespeeds(GridD g, Dev d, kz_physics ph) {
+__global__ void __launch_bounds__(128) k_wavespeeds(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   int act[3] = {g.nx > 1, g.ny > 1, g.nz > 1};
   long lx = act[0], ly = act[1], lz = act[2];
@@ -612,7 +629,7 @@ __global__ void k_ct_rebuild(GridD g, Dev d) {
 // ---------------------------------------------------------------------------
 // conserved update: flux divergence + metric source (sim.zig:1140-1175,
 // 1542-1582). dt is read from the step scalars (slot `sslot`).
-__global__ void k_update(GridD g, Dev d, kz_physics ph, int sslot) {
+__global__ void __launch_bounds__(128) k_update(GridD g, Dev d, kz_physics ph, int sslot) {
   long t = TID;
   if (t >= (long)g.nx * g.ny * g.nz) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
@@ -673,31 +690,145 @@ __global__ void k_update(GridD g, Dev d, kz_physics ph, int sslot) {
 }
 
 // ---------------------------------------------------------------------------
-// implicit four-force solve (sim.zig:1736-1776); dt from step scalars
-__global__ void __launch_bounds__(64) k_implicit(GridD g, Dev d, kz_physics ph, int sslot) {
+// implicit four-force solve (sim.zig:1736-1776 -> implicit.zig:927-1004),
+// split in three kernels so that each keeps its state in registers:
+//   k_imp_prep     reference state, bisection warm start, starting block
+//   k_imp_rung0<W> the first rung (energy equation, lab frame) for the
+//                  cells whose starting primitive block is W
+//   k_imp_slow     rungs 1..5 (and damped-opacity levels) for the cells the
+//                  first rung did not solve, gathered into a list
+// The sequence of attempts per cell is the reference ladder's; inputs to
+// every rung are the stored reference-state values, so results equal the
+// single-pass ladder bit for bit.
+__device__ __forceinline__ long dom_t(const GridD &g, long ix, long iy, long iz) {
+  return ix + (long)g.nx * (iy + (long)g.ny * iz);
+}
+__global__ void __launch_bounds__(64) k_imp_prep(GridD g, Dev d, kz_physics ph, int sslot) {
   long t = TID;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
+  if (t == 0) *d.imp_nslow = 0;  // no appends run concurrently with this kernel
+  long nd = (long)g.nx * g.ny * g.nz;
+  if (t >= nd) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
   if (polar_row(ph, g, iy)) return;
   long n = g.ncell;
   long c = cidx(g, ix, iy, iz);
   double dt = d.ss[sslot];
-  double uu[NV], pp[NV];
-  ld(d.u, n, c, uu);
-  ld(d.p, n, c, pp);
+  double pp00[NV], uu00[NV], pp0[NV];
+  ld(d.p, n, c, pp00);
   Geo gm;
   geo_c(g, d, ix, iy, iz, gm);
-  kz_implicit_result r = solve_implicit_lab(uu, pp, gm, dt, ph.gam, ph.rad, ph.opac, ph.implicit);
-  if (r.ok) {
-    st(d.u, n, c, uu);
-    st(d.p, n, c, pp);
-    d.flags[KZ_FLAG_RADIMP_FIXUP * n + c] = 0;
-    atomicAdd(&d.cnt[1], (unsigned long long)r.iters);
-    atomicAdd(&d.cnt[2], 1ULL);
+  p2u(pp00, uu00, gm, ph.gam);
+  double ehat = -calc_ff_rtt(pp00, gm);
+  RadState st00;
+  fill_radstate(pp00, gm, ph.gam, ph.opac, true, st00);
+  ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+  if (ph.implicit.start_with_bisect) {
+    solve_1d_prim(pp00, st00.ehat, st00.ucon[0], st00.tgas, dt, cx, pp0);
   } else {
-    d.flags[KZ_FLAG_RADIMP_FIXUP * n + c] = -1;
-    atomicAdd(&d.cnt[0], 1ULL);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
   }
+  int sw = (ehat < ph.implicit.threshold * pp00[UU]) ? WP_RAD : WP_MHD;
+  st(d.imp_uu00, nd, t, uu00);
+  st(d.imp_pp0, nd, t, pp0);
+  double *a = d.imp_aux;
+  a[IA_EHAT * nd + t] = st00.ehat;
+  a[IA_UINT * nd + t] = st00.uint;
+  a[IA_SGAS * nd + t] = st00.sgas;
+  a[IA_KES * nd + t] = st00.kappaes;
+  a[IA_KGA * nd + t] = st00.op.gas_abs;
+  a[IA_KRA * nd + t] = st00.op.rad_abs;
+  a[IA_KRR * nd + t] = st00.op.rad_ross;
+  a[IA_SW * nd + t] = (double)sw;
+}
+__device__ __forceinline__ void imp_load(const Dev &d, long nd, long t, double *uu00, double *pp0, State0 &s0,
+                                         LagOpac &lag) {
+  ld(d.imp_uu00, nd, t, uu00);
+  ld(d.imp_pp0, nd, t, pp0);
+  const double *a = d.imp_aux;
+  s0.ehat = a[IA_EHAT * nd + t];
+  s0.uint = a[IA_UINT * nd + t];
+  s0.sgas = a[IA_SGAS * nd + t];
+  lag.kappaes = a[IA_KES * nd + t];
+  lag.op.gas_abs = a[IA_KGA * nd + t];
+  lag.op.rad_abs = a[IA_KRA * nd + t];
+  lag.op.rad_ross = a[IA_KRR * nd + t];
+}
+// store a solved cell (sim.zig:1762-1768)
+__device__ __forceinline__ void imp_commit(const GridD &g, const Dev &d, const kz_physics &ph, long c, const Geo &gm,
+                                           const double *ppw, int iters) {
+  double uu[NV];
+  p2u(ppw, uu, gm, ph.gam);
+  st(d.u, g.ncell, c, uu);
+  st(d.p, g.ncell, c, ppw);
+  d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + c] = 0;
+  atomicAdd(&d.cnt[1], (unsigned long long)iters);
+  atomicAdd(&d.cnt[2], 1ULL);
+}
+template <int WP>
+__global__ void __launch_bounds__(64) k_imp_rung0(GridD g, Dev d, kz_physics ph, int sslot) {
+  long t = TID;
+  long nd = (long)g.nx * g.ny * g.nz;
+  if (t >= nd) return;
+  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  if (polar_row(ph, g, iy)) return;
+  if ((int)d.imp_aux[IA_SW * nd + t] != WP) return;
+  double uu00[NV], pp0[NV], ppw[NV];
+  State0 s0;
+  LagOpac lag;
+  imp_load(d, nd, t, uu00, pp0, s0, lag);
+  Geo gm;
+  geo_c(g, d, ix, iy, iz, gm);
+  double dt = d.ss[sslot];
+  ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+  int iters = 0;
+  bool ok = ph.implicit.opdamp_maxlevels >= 0 &&
+            solve_4d_prim_core<WP>(uu00, lag, s0, pp0, dt, cx, WE_ENERGY, WF_LAB, ppw, &iters);
+  if (ok) {
+    imp_commit(g, d, ph, cidx(g, ix, iy, iz), gm, ppw, iters);
+  } else {
+    unsigned long long k = atomicAdd(d.imp_nslow, 1ULL);
+    d.imp_slow[k] = (int32_t)t;
+  }
+}
+__global__ void __launch_bounds__(64) k_imp_slow(GridD g, Dev d, kz_physics ph, int sslot) {
+  long i = TID;
+  if (i >= (long)*d.imp_nslow) return;
+  long nd = (long)g.nx * g.ny * g.nz;
+  long t = d.imp_slow[i];
+  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  long c = cidx(g, ix, iy, iz);
+  double uu00[NV], pp0[NV];
+  State0 s0;
+  LagOpac lag;
+  imp_load(d, nd, t, uu00, pp0, s0, lag);
+  const int sw = (int)d.imp_aux[IA_SW * nd + t];
+  const int ot = (sw == WP_RAD) ? WP_MHD : WP_RAD;
+  Geo gm;
+  geo_c(g, d, ix, iy, iz, gm);
+  double dt = d.ss[sslot];
+  const kz_implicit_params &ip = ph.implicit;
+  ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ip, 1.0};
+  for (int level = 0; level <= ip.opdamp_maxlevels; ++level) {
+    double od = 1.0;
+    for (int k = 0; k < level; ++k) od /= ip.opdamp_factor;
+    cx.opdamp = od;
+    for (int ir = level == 0 ? 1 : 0; ir < 6; ++ir) {
+      int wp = (ir == 0 || ir == 1 || ir == 4) ? sw : ot;
+      int we = (ir >= 4) ? WE_ENTROPY : WE_ENERGY;
+      int wf = (ir == 0 || ir == 2) ? WF_LAB : WF_FF;
+      double ppw[NV];
+      int iters = 0;
+      bool ok = (wp == WP_MHD) ? solve_4d_prim_core<WP_MHD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters)
+                               : solve_4d_prim_core<WP_RAD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters);
+      if (ok) {
+        imp_commit(g, d, ph, c, gm, ppw, iters);
+        return;
+      }
+    }
+  }
+  d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + c] = -1;
+  atomicAdd(&d.cnt[0], 1ULL);
 }
 
 // ---------------------------------------------------------------------------
@@ -997,6 +1128,12 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   e |= dalloc(s, (void **)&D.tsd, 2 * sizeof(unsigned long long));
   e |= dalloc(s, (void **)&D.err, sizeof(int));
   e |= dalloc(s, (void **)&D.ss, SS_N * sizeof(double));
+  long nd = (long)gr->nx * gr->ny * gr->nz;
+  e |= dalloc(s, (void **)&D.imp_uu00, (size_t)nd * NV * sizeof(double));
+  e |= dalloc(s, (void **)&D.imp_pp0, (size_t)nd * NV * sizeof(double));
+  e |= dalloc(s, (void **)&D.imp_aux, (size_t)nd * IMP_NAUX * sizeof(double));
+  e |= dalloc(s, (void **)&D.imp_slow, (size_t)nd * sizeof(int32_t));
+  e |= dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long));
   if (e) { *err = e < 0 ? e : KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // metric tables
   std::vector<double> buf;
@@ -1108,8 +1245,8 @@ int kz_sim_p2u_domain(kz_sim *s) {
 static void launch_set_bc(kz_sim *s) {
   const GridD &g = s->g;
   TimedLaunch tl(s, KZ_K_BC);
-  k_bc_x<<<nblk((long)g.nz * g.ny * g.ngx * 2, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
-  k_bc_y<<<nblk((long)g.nz * g.nx * g.ngy * 2, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
+  long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2;
+  k_bc_xy<<<nblk(nb, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
   k_bc_corners2d<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
 }
 
@@ -1162,7 +1299,11 @@ static void launch_op_implicit(kz_sim *s, int dt_slot) {
   if (!s->ph.has_opac) return;
   {
     TimedLaunch tl(s, KZ_K_IMPLICIT);
-    k_implicit<<<nblk(ndom(s), 64), 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
+    unsigned nb = nblk(ndom(s), 64);
+    k_imp_prep<<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
+    k_imp_rung0<WP_MHD><<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
+    k_imp_rung0<WP_RAD><<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
+    k_imp_slow<<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
   }
   if (s->ph.do_fixups && s->ph.do_radimp_fixups)
     k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);

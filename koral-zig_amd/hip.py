@@ -53,7 +53,8 @@ _SIGS = {
     "kz_solve_implicit_lab_batch": (C.c_int, [C.c_int64, D, D, GEOM, D, C.c_double, C.POINTER(T.kz_rad_params),
                                               C.POINTER(T.kz_opac_params), C.POINTER(T.kz_implicit_params),
                                               C.c_void_p]),
-    "kz_ppm_batch": (C.c_int, [C.c_int64, D, D, D, D]),
+    "kz_reconstruct_batch": (C.c_int, [C.c_int64, I32, D, D, D, D, D]),
+    "kz_u2p_solver_w_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.c_int32, C.c_double, I32]),
     "kz_flux_prime_batch": (C.c_int, [C.c_int64, D, I32, GEOM, C.c_double, D, I32]),
     "kz_gas_wavespeeds_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, I32, D]),
     "kz_rad_wavespeeds_batch": (C.c_int, [C.c_int64, D, GEOM, D, I32, D]),
@@ -255,12 +256,23 @@ def solve_implicit(uu, pp, geom, dt, gam, rad, opac, imp):
     return uu, pp, r
 
 
-def ppm(u5, dx5):
+def reconstruct(order, theta, u5, dx5):
     u5, dx5 = _f64(u5), _f64(dx5)
     n = len(u5)
+    order = np.ascontiguousarray(np.broadcast_to(np.asarray(order, np.int32), (n,)))
+    theta = _f64(np.broadcast_to(np.asarray(theta, np.float64), (n,)))
     l, r = np.zeros(n), np.zeros(n)
-    _chk(lib().kz_ppm_batch(n, T.ptr(u5), T.ptr(dx5), T.ptr(l), T.ptr(r)), "kz_ppm_batch")
+    _chk(lib().kz_reconstruct_batch(n, T.ptr(order, C.c_int32), T.ptr(theta), T.ptr(u5), T.ptr(dx5), T.ptr(l),
+                                    T.ptr(r)), "kz_reconstruct_batch")
     return l, r
+
+
+def u2p_solver_w(uu, guess, geom, etype, gam):
+    uu, pp = _f64(uu), _f64(guess).copy()
+    ret = np.zeros(len(uu), np.int32)
+    _chk(lib().kz_u2p_solver_w_batch(len(uu), T.ptr(uu), T.ptr(pp), T.ptr(_geoms(geom)), int(etype), gam,
+                                     T.ptr(ret, C.c_int32)), "kz_u2p_solver_w_batch")
+    return ret, pp
 
 
 def flux_prime(pp, idim, geom, gam):

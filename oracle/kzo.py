@@ -61,6 +61,8 @@ def lib():
         L.kzo_sim_get_scal.restype = None
         L.kzo_sim_get_flb.argtypes = [vp, C.c_int, _D]
         L.kzo_sim_get_flb.restype = None
+        L.kzo_sim_set_flb.argtypes = [vp, C.c_int, _D]
+        L.kzo_sim_set_flb.restype = None
         L.kzo_sim_counters.argtypes = [vp, C.POINTER(T.kz_counters)]
         L.kzo_sim_counters.restype = None
         L.kzo_sim_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_double]
@@ -294,6 +296,23 @@ class Sim:
         f = np.zeros(self.shape + (T.NFLAGS,), np.int32)
         lib().kzo_sim_get_state(self.h, T.ptr(p), T.ptr(u), T.ptr(f, C.c_int32))
         return p, u, f
+
+    def face_shape(self, dim):
+        (sz, sy, sx) = self.shape
+        return (sz + (dim == 2), sy + (dim == 1), sx + (dim == 0), T.NV)
+
+    def get_flb(self, dim):
+        f = np.zeros(self.face_shape(dim))
+        lib().kzo_sim_get_flb(self.h, dim, T.ptr(f))
+        return f
+
+    def set_flb(self, dim, f):
+        f = np.ascontiguousarray(f, np.float64)
+        assert f.shape == self.face_shape(dim)
+        lib().kzo_sim_set_flb(self.h, dim, T.ptr(f))
+
+    def flux_ct(self):
+        lib().kzo_flux_ct(self.h)
 
     def get_scal(self):
         s = np.zeros(self.shape + (20,))

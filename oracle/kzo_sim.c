@@ -168,6 +168,7 @@ void kzo_sim_get_state(kzo_sim *s, double *p, double *u, int32_t *flags) {
 }
 void kzo_sim_get_scal(kzo_sim *s, double *scal) { memcpy(scal, s->scal, (size_t)s->ncell * NSCAL * sizeof(double)); }
 void kzo_sim_get_flb(kzo_sim *s, int dim, double *out) { memcpy(out, s->flb[dim], (size_t)s->nface[dim] * NV * sizeof(double)); }
+void kzo_sim_set_flb(kzo_sim *s, int dim, const double *in) { memcpy(s->flb[dim], in, (size_t)s->nface[dim] * NV * sizeof(double)); }
 void kzo_sim_counters(kzo_sim *s, kz_counters *c) {
   c->t = s->t; c->dt = s->dt; c->own_dt = s->own_dt; c->tstepdenmax = s->tstepdenmax; c->tstepdenmin = s->tstepdenmin;
   c->nstep = s->nstep; c->n_radimp_failures = s->n_fail; c->n_radimp_iters = s->n_iters;

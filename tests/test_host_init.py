@@ -114,3 +114,26 @@ def test_perturbation_is_seeded_and_bounded():
     assert (np.abs(dom[..., [0, 1, 2, 3, 4, 9, 10, 11, 12]] - 1.0) < 1e-4).all()
     xs = np.array([host.uniform(7, i, 3) for i in range(20000)])
     assert xs.min() >= -1.0 and xs.max() < 1.0 and abs(xs.mean()) < 0.02
+
+
+def test_calc_b_from_a_golden():
+    """step_golden_tests.zig:340-380 — calc_BfromA (corner average + curl,
+    overwrite) on a periodic 32x32 Minkowski grid vs C, 1e-14."""
+    inp, out = read_kgld("flux/bfroma.kgld", 5, 3)
+    tp = 2.0 * np.pi
+    grid = T.kz_grid(nx=32, ny=32, nz=1, ng=3, minx=0.0, maxx=tp, miny=0.0, maxy=tp, minz=0.0, maxz=1.0)
+    (sz, sy, sx), (ngz, ngy, ngx) = T.padded_shape(grid)
+    a = np.zeros((32, 32, 3))
+    for r in inp:
+        a[int(r[1]), int(r[0])] = r[2:5]
+    # periodic ghosts in both dimensions (bc_x = bc_y = periodic)
+    iy = (np.arange(sy) - ngy) % 32
+    ix = (np.arange(sx) - ngx) % 32
+    p = np.zeros((sz, sy, sx, T.NV))
+    p[0, :, :, 6:9] = a[iy][:, ix]
+    g20 = np.zeros((sz * sy * sx, 20))
+    g20[:, 19] = 1.0  # Minkowski: sqrt(-g) = 1
+    p = host.calc_b_from_a(grid, {"g": g20.ravel()}, p)
+    got = np.array([p[0, int(r[1]) + ngy, int(r[0]) + ngx, 6:9] for r in inp])
+    t = DevTracker().add(out, got)
+    assert t.ok(1e-14), t.max_dev

@@ -278,3 +278,32 @@ def test_implicit_golden(P):
             sc = vmax if iv in vel else max(abs(cp[iv]), abs(pp[i, iv]), 1e-300)
             worst = max(worst, abs(cp[iv] - pp[i, iv]) / sc)
     assert worst <= 1e-6, worst
+
+
+def test_flux_ct_golden():
+    """step_golden_tests.zig:299-336 — flux-CT (Toth EMF averaging) on a 32x32
+    grid: the B rows of the face fluxes set from the records, fluxCt, every
+    record's rebuilt B rows compared at 1e-15."""
+    from koral_zig_amd import host
+    inp, out = read_kgld("flux/ct.kgld", 6, 3)
+    tp = 2.0 * np.pi
+    grid = T.kz_grid(nx=32, ny=32, nz=1, ng=3, minx=0.0, maxx=tp, miny=0.0, maxy=tp, minz=0.0, maxz=1.0)
+    sim = kzo.Sim(grid, T.physics_puffy(), host.metric_tables(grid))
+    flb = [sim.get_flb(d) for d in range(2)]
+    ng = 3
+    for r in inp:
+        d, ix, iy = int(r[0]), int(r[1]), int(r[2])
+        flb[d][0, iy + ng, ix + ng, 6:9] = r[3:6]
+    for d in range(2):
+        sim.set_flb(d, flb[d])
+    sim.flux_ct()
+    res = [sim.get_flb(d) for d in range(2)]
+    got = np.array([res[int(r[0])][0, int(r[2]) + ng, int(r[1]) + ng, 6:9] for r in inp])
+    t = DevTracker().add(out, got)
+    assert t.ok(1e-15), t.max_dev
+    # the normal B flux of every rebuilt face is zeroed (ct.zig:160,166)
+    dims, ix, iy = inp[:, 0].astype(int), inp[:, 1].astype(int), inp[:, 2].astype(int)
+    rx = (dims == 0) & (iy >= 0) & (iy < 32)
+    ry = (dims == 1) & (ix >= 0) & (ix < 32)
+    assert rx.sum() == 33 * 32 and ry.sum() == 33 * 32
+    assert (got[rx, 0] == 0).all() and (got[ry, 1] == 0).all()
