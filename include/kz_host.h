@@ -36,6 +36,11 @@ int kz_host_dxdx(int dir, double a, double mksr0, double mksh0, const double *x4
  * r_bound J_mks2->bl[16] J_bl->mks2[16]. */
 int kz_host_bc_xhi_tables(const kz_grid *g, double a, double mksr0, double mksh0, double *out);
 
+/* The BL sidecar of MetricCache (koral/metric/precompute.zig:361-380,
+ * :457-475) for the radiative viscosity and the dynamo: 52 doubles per padded
+ * cell, gBL[16] GBL[16] alpha_BL gdet_BL r theta J_mks2->bl[16]. */
+int kz_host_bl_tables(const kz_grid *g, double a, double mksr0, double mksh0, double *out);
+
 /* PUFFY prepinit + init over the domain (puffy.zig:715-866): padded AoS
  * primitives with A_phi in the B3 slot and ENTR = S(rho, u). */
 int kz_host_puffy_prepinit(const kz_grid *g, double a, double mksr0, double mksh0, double gam,

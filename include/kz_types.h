@@ -126,6 +126,19 @@ typedef struct kz_opac_params {
   int32_t _pad;
 } kz_opac_params;
 
+/* radiative shear viscosity knobs (koral/physics/radvisc.zig:44-53):
+ * nu = alpha * mfp; the viscous flux is damped to speed <= maxvel */
+typedef struct kz_radvisc_params {
+  double alpha;
+  double maxvel;
+} kz_radvisc_params;
+
+/* mean-field "mimic" dynamo knobs (koral/sim/dynamo.zig:54-64) */
+typedef struct kz_dynamo_params {
+  double alphadynamo, thetaangle, alphabeta, betasaturated, expectedhr;
+  int32_t alphaflipssign, dampbeta, calchronthego, _pad;
+} kz_dynamo_params;
+
 /* Everything the per-cell kernels read (Sim.Options subset, koral/sim.zig:
  * 130-204, as set by Physics.toOptions, koral/problems/puffy/puffy.zig:210-242). */
 typedef struct kz_physics {
@@ -145,9 +158,12 @@ typedef struct kz_physics {
   int32_t reduceorderatbh;        /* must be 0 */
   int32_t reduceorderafterfixup;  /* must be 0 */
   int32_t dampradwavespeednearaxis;
-  int32_t radviscosity;           /* must be 0 in this build */
-  int32_t dynamo;                 /* must be 0 in this build */
+  int32_t radviscosity;           /* radiative shear viscosity (needs the BL tables) */
+  int32_t dynamo;                 /* mean-field dynamo after each explicit stage (BL tables) */
   int32_t _pad;
+  kz_radvisc_params radvisc;
+  kz_dynamo_params dynamo_params;
+  double bh_spin;                 /* metric spin a (Options.mp.a) */
 } kz_physics;
 
 /* Grid (koral/grid.zig:10-161): uniform internal coordinates, NG ghosts on

@@ -63,6 +63,16 @@ class kz_opac_params(C.Structure):
             "synchrotron_bridge", "kappa_mode", "kappaes_mode", "_pad")]
 
 
+class kz_radvisc_params(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("maxvel", C.c_double)]
+
+
+class kz_dynamo_params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("alphadynamo", "thetaangle", "alphabeta", "betasaturated",
+                                          "expectedhr")] + \
+        [(n, C.c_int32) for n in ("alphaflipssign", "dampbeta", "calchronthego", "_pad")]
+
+
 class kz_physics(C.Structure):
     _fields_ = [("gam", C.c_double), ("tsteplim", C.c_double), ("minmod_theta", C.c_double),
                 ("floors", kz_floor_params), ("rad", kz_rad_params), ("opac", kz_opac_params),
@@ -70,7 +80,8 @@ class kz_physics(C.Structure):
         [(n, C.c_int32) for n in (
             "has_opac", "do_fixups", "do_u2prad_fixups", "do_radimp_fixups", "correct_polaraxis",
             "nccorrectpolar", "reduceorderatbh", "reduceorderafterfixup", "dampradwavespeednearaxis",
-            "radviscosity", "dynamo", "_pad")]
+            "radviscosity", "dynamo", "_pad")] + \
+        [("radvisc", kz_radvisc_params), ("dynamo_params", kz_dynamo_params), ("bh_spin", C.c_double)]
 
 
 class kz_grid(C.Structure):
@@ -170,15 +181,30 @@ def opac_params_puffy(mass=10.0):
         kappa_mode=0, kappaes_mode=1)
 
 
-def physics_puffy(mass=10.0):
-    """The per-cell knobs of Physics.toOptions (koral/problems/puffy/puffy.zig:210-242)
-    with radiative viscosity and the mimic dynamo off."""
+def radvisc_params_puffy():
+    """radvisc.Params defaults = PUFFY define.h:96-102 (koral/physics/radvisc.zig:44-53)."""
+    return kz_radvisc_params(alpha=0.1, maxvel=0.1)
+
+
+def dynamo_params_puffy():
+    """dynamo.Params defaults = PUFFY define.h:66-74 (koral/sim/dynamo.zig:54-64)."""
+    return kz_dynamo_params(alphadynamo=2.0 * 0.314, thetaangle=0.25, alphabeta=2.0 * 6.28,
+                            betasaturated=0.1, expectedhr=0.3, alphaflipssign=1, dampbeta=1,
+                            calchronthego=1)
+
+
+def physics_puffy(mass=10.0, radviscosity=False, dynamo=False):
+    """The per-cell knobs of Physics.toOptions (koral/problems/puffy/puffy.zig:210-242).
+    Radiative viscosity and the mimic dynamo (on in PUFFY production runs) are
+    off unless asked for."""
     return kz_physics(gam=5.0 / 3.0, tsteplim=0.5, minmod_theta=1.5,
                       floors=floor_params_puffy(), rad=rad_params_puffy(),
                       opac=opac_params_puffy(mass), implicit=implicit_params_puffy(),
                       has_opac=1, do_fixups=1, do_u2prad_fixups=0, do_radimp_fixups=0,
                       correct_polaraxis=1, nccorrectpolar=2, reduceorderatbh=0,
-                      reduceorderafterfixup=0, dampradwavespeednearaxis=0, radviscosity=0, dynamo=0)
+                      reduceorderafterfixup=0, dampradwavespeednearaxis=0,
+                      radviscosity=int(radviscosity), dynamo=int(dynamo),
+                      radvisc=radvisc_params_puffy(), dynamo_params=dynamo_params_puffy(), bh_spin=0.0)
 
 
 def grid_puffy(nx, ny, nz=1, rmin=1.85, rmax=500.0, mksr0=0.1):

@@ -44,6 +44,7 @@ def lib():
         L.kz_host_perturb.argtypes = [G, C.c_uint64, C.c_double, C.c_double, C.c_int, D]
         L.kz_host_geometry_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_long, C.c_long, C.c_long,
                                           C.c_void_p]
+        L.kz_host_bl_tables.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
         _lib = L
     return _lib
 
@@ -147,3 +148,15 @@ def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-
         engine.set_bc(False)
     engine.init_timestep_guess()
     return fac
+
+
+BL_REC = 52  # gBL[16] GBL[16] alpha gdet r theta J_mks2->bl[16]
+
+
+def bl_tables(grid, mp=MP_PUFFY):
+    """The BL sidecar (MetricCache.blGeom / jacMy2Bl) per padded cell."""
+    L = lib()
+    nc = L.kz_host_cell_count(C.byref(grid))
+    out = np.zeros(nc * BL_REC)
+    L.kz_host_bl_tables(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"], T.ptr(out))
+    return out

@@ -1127,6 +1127,37 @@ int kz_host_perturb(const kz_grid *gr, uint64_t seed, double amp, double gam, in
 }
 
 // geometry of one point in BL (for tests of the xhi tables / init)
+// The BL sidecar of MetricCache (precompute.zig:361-380, :457-475) over every
+// padded cell: 52 doubles per cell = gBL[16] GBL[16] alpha_BL gdet_BL r theta
+// J_mks2->bl[16].  The radiative viscosity reads r, the dynamo all of it.
+int kz_host_bl_tables(const kz_grid *gr, double a, double mksr0, double mksh0, double *out) {
+  Grid G(*gr);
+  MP mp{a, mksr0, mksh0};
+  const long nrows = (long)G.sz * G.sy;
+#pragma omp parallel for schedule(static)
+  for (long row = 0; row < nrows; ++row) {
+    long iz = row / G.sy - G.ngz, iy = row % G.sy - G.ngy;
+    for (long ix = -G.ngx; ix < gr->nx + G.ngx; ++ix) {
+      double *o = out + G.cidx(ix, iy, iz) * 52;
+      kz_geom b = geometry_bl_at(G, mp, ix, iy, iz);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+          o[i * 4 + j] = b.gg[i][j];
+          o[16 + i * 4 + j] = b.GG[i][j];
+        }
+      o[32] = b.alpha;
+      o[33] = b.gdet;
+      o[34] = b.xx[1];
+      o[35] = b.xx[2];
+      double x[4] = {0.0, G.xc(ix), G.yc(iy), G.zc(iz)}, J[4][4];
+      dxdx_mks2_to_bl(x, mp, J);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) o[36 + i * 4 + j] = J[i][j];
+    }
+  }
+  return 0;
+}
+
 int kz_host_geometry_bl(const kz_grid *gr, double a, double mksr0, double mksh0, long ix, long iy, long iz,
                         kz_geom *out) {
   Grid G(*gr);

@@ -76,6 +76,10 @@ enum { WP_RAD = 1, WP_MHD = 2 };
 enum { WE_ENERGY = 0, WE_ENTROPY = 1 };
 enum { WF_LAB = 0, WF_FF = 1 };
 
+/* ---- metric.zig:20-32 ---- */
+double kzo_r_horizon_bl(double a);
+double kzo_r_isco_bl(double a);
+
 /* ---- geometry ---- */
 void kzo_geo_from_kz(const kz_geom *k, kzo_geo *o);
 void kzo_geo_from_blocks(const double *g20, const double *G20, double x1, kzo_geo *o);

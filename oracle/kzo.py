@@ -63,6 +63,32 @@ def lib():
         L.kzo_sim_get_flb.restype = None
         L.kzo_sim_set_flb.argtypes = [vp, C.c_int, _D]
         L.kzo_sim_set_flb.restype = None
+        L.kzo_sim_set_bl_tables.argtypes = [vp, _D]
+        L.kzo_sim_set_bl_tables.restype = None
+        L.kzo_sim_calc_rij_visc_total.argtypes = [vp, C.c_double]
+        L.kzo_sim_get_rijvisc.argtypes = [vp, _D]
+        L.kzo_sim_get_rijvisc.restype = None
+        L.kzo_sim_apply_dynamo.argtypes = [vp, C.c_double]
+        L.kzo_sim_get_scaleth.argtypes = [vp, _D]
+        L.kzo_sim_get_scaleth.restype = None
+        L.kzo_step_function.argtypes = [C.c_double, C.c_double]
+        L.kzo_step_function.restype = C.c_double
+        L.kzo_visc_coeff.argtypes = [C.c_double] * 6
+        L.kzo_visc_coeff.restype = C.c_double
+        L.kzo_shear_from_gradients.argtypes = [_D, _D, _D, _D, _D, _D, _D]
+        L.kzo_shear_from_gradients.restype = None
+        L.kzo_dynamo_delta_a.argtypes = [C.POINTER(T.kz_dynamo_params)] + [C.c_double] * 12 + [_D]
+        L.kzo_dynamo_delta_a.restype = C.c_double
+        L.kzo_damp_bphi.argtypes = [C.c_double] * 7
+        L.kzo_damp_bphi.restype = C.c_double
+        L.kzo_sim_shear_lab.argtypes = [vp, C.c_long, C.c_long, C.c_long, _D]
+        L.kzo_sim_get_dyn_a.argtypes = [vp, _D]
+        L.kzo_sim_get_dyn_a.restype = None
+        L.kzo_sim_mimic_dynamo.argtypes = [vp, C.c_double]
+        L.kzo_r_horizon_bl.argtypes = [C.c_double]
+        L.kzo_r_horizon_bl.restype = C.c_double
+        L.kzo_r_isco_bl.argtypes = [C.c_double]
+        L.kzo_r_isco_bl.restype = C.c_double
         L.kzo_sim_counters.argtypes = [vp, C.POINTER(T.kz_counters)]
         L.kzo_sim_counters.restype = None
         L.kzo_sim_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_double]
@@ -314,6 +340,35 @@ class Sim:
     def flux_ct(self):
         lib().kzo_flux_ct(self.h)
 
+    def set_bl_tables(self, bl):
+        """BL sidecar for radiative viscosity / dynamo (kept referenced)."""
+        self._bl = np.ascontiguousarray(bl, np.float64)
+        lib().kzo_sim_set_bl_tables(self.h, T.ptr(self._bl))
+
+    def calc_rij_visc_total(self, global_dt):
+        self._chk(lib().kzo_sim_calc_rij_visc_total(self.h, global_dt), "calc_rij_visc_total")
+        r = np.zeros(self.shape + (16,))
+        lib().kzo_sim_get_rijvisc(self.h, T.ptr(r))
+        return r
+
+    def shear_lab(self, ix, iy, iz=0):
+        out = np.zeros(16)
+        lib().kzo_sim_shear_lab(self.h, ix, iy, iz, T.ptr(out))
+        return out.reshape(4, 4)
+
+    def mimic_dynamo(self, dt):
+        """Scale height, setBc, mimicDynamo (no final calcU2p); returns dyn_a."""
+        self._chk(lib().kzo_sim_mimic_dynamo(self.h, dt), "mimic_dynamo")
+        a = np.zeros(self.shape)
+        lib().kzo_sim_get_dyn_a(self.h, T.ptr(a))
+        return a
+
+    def apply_dynamo(self, dt):
+        self._chk(lib().kzo_sim_apply_dynamo(self.h, dt), "apply_dynamo")
+        h = np.zeros(self.grid.nx)
+        lib().kzo_sim_get_scaleth(self.h, T.ptr(h))
+        return h
+
     def get_scal(self):
         s = np.zeros(self.shape + (20,))
         lib().kzo_sim_get_scal(self.h, T.ptr(s))
@@ -371,3 +426,39 @@ def true_residual(pp, pp00, dt, geom, gam, op, wp, we, wf):
                                   C.c_double(gam), C.byref(op), T.ptr(wp, C.c_int32), T.ptr(we, C.c_int32),
                                   T.ptr(wf, C.c_int32), T.ptr(err))
     return err
+
+
+def step_function(x, x9):
+    return lib().kzo_step_function(x, x9)
+
+
+def visc_coeff(chi, mindx, r_bl, rhor, alpha, global_dt):
+    return lib().kzo_visc_coeff(chi, mindx, r_bl, rhor, alpha, global_dt)
+
+
+def shear_from_gradients(du, du2, ucon, ucov, geom, kr):
+    """radvisc.zig shearFromGradients over one point; geom a kz_geom row."""
+    out = np.zeros(16)
+    a = [np.ascontiguousarray(x, np.float64).ravel() for x in (du, du2, ucon, ucov, geom, kr)]
+    lib().kzo_shear_from_gradients(*[T.ptr(x) for x in a], T.ptr(out))
+    return out.reshape(4, 4)
+
+
+def dynamo_delta_a(dp, a, dt, rhor, risco, r, th, bphi, bsq, angle, prermhd, gg33, scaleth):
+    """dynamo.zig dynamoDeltaA: returns (aphi, new bphi)."""
+    b = np.zeros(1)
+    aphi = lib().kzo_dynamo_delta_a(C.byref(dp), a, dt, rhor, risco, r, th, bphi, bsq, angle, prermhd, gg33,
+                                    scaleth, T.ptr(b))
+    return aphi, b[0]
+
+
+def r_horizon_bl(a):
+    return lib().kzo_r_horizon_bl(a)
+
+
+def r_isco_bl(a):
+    return lib().kzo_r_isco_bl(a)
+
+
+def damp_bphi(alphabeta, facradius, faczh, dt_over_pk, beta, betasat, bphi):
+    return lib().kzo_damp_bphi(alphabeta, facradius, faczh, dt_over_pk, beta, betasat, bphi)

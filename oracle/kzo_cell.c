@@ -23,6 +23,18 @@ void kzo_geo_from_kz(const kz_geom *k, kzo_geo *o) {
   o->x1 = k->xx[1];
 }
 
+/* metric.zig:20-22 rHorizonBL */
+double kzo_r_horizon_bl(double a) { return 1.0 + kzm_sqrt(1.0 - a * a); }
+
+/* metric.zig:26-32 rIscoBL.  The reference's pow takes y == 1/2 as a square
+ * root (std.math.pow's special case); the cube roots go through kzm_pow. */
+double kzo_r_isco_bl(double ac) {
+  const double c3 = 1.0 / 3.0;
+  double z1 = 1.0 + kzm_pow(1.0 - ac * ac, c3) * (kzm_pow(1.0 + ac, c3) + kzm_pow(1.0 - ac, c3));
+  double z2 = kzm_sqrt(3.0 * ac * ac + z1 * z1);
+  return 3.0 + z2 - kzm_sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
 /* precompute.zig:466-496: alpha = sqrt(-1/GG00) recomputed from the block */
 void kzo_geo_from_blocks(const double *g20, const double *G20, double x1, kzo_geo *o) {
   for (int i = 0; i < 4; ++i)

@@ -47,7 +47,16 @@ typedef struct kzo_sim {
   double *emf[3];
   double t, dt, own_dt, tstepdenmax, tstepdenmin, min_dx, min_dy, min_dz;
   uint64_t nstep, n_fail, n_iters, n_solves, n_guard;
+  /* radiative viscosity and dynamo state */
+  const double *bl;   /* BL sidecar, BL_REC per padded cell (caller-owned) */
+  double *rijvisc;    /* R^i_j viscous, 16 per padded cell */
+  double *dyn_a;      /* Delta A_phi per padded cell */
+  double *vecpot;     /* 6 per padded cell: corner A (0..2), curl B (3..5) */
+  double *scaleth;    /* nx */
+  double rhor, risco;
 } kzo_sim;
+
+#define BL_REC 52 /* gBL[16] GBL[16] alpha gdet r theta J_mks2->bl[16] */
 
 /* ---- grid (koral/grid.zig:127-159) ---- */
 static double xl_(const kzo_sim *s, long i) { return s->grid.minx + (double)i * s->dx; }
@@ -55,6 +64,7 @@ static double yl_(const kzo_sim *s, long i) { return s->grid.miny + (double)i * 
 static double zl_(const kzo_sim *s, long i) { return s->grid.minz + (double)i * s->dz; }
 static double xc_(const kzo_sim *s, long i) { return 0.5 * (xl_(s, i) + xl_(s, i + 1)); }
 static double yc_(const kzo_sim *s, long i) { return 0.5 * (yl_(s, i) + yl_(s, i + 1)); }
+static double zc_(const kzo_sim *s, long i) { return 0.5 * (zl_(s, i) + zl_(s, i + 1)); }
 static double csize(const kzo_sim *s, long i, int dim) {
   if (dim == 0) return xl_(s, i + 1) - xl_(s, i);
   if (dim == 1) return yl_(s, i + 1) - yl_(s, i);
@@ -96,8 +106,9 @@ kzo_sim *kzo_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g
                         const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
   if (gr->nz != 1 || gr->ny <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return 0; }
-  if (ph->radviscosity || ph->dynamo || ph->reduceorderatbh || ph->reduceorderafterfixup ||
+  if (ph->reduceorderatbh || ph->reduceorderafterfixup ||
       ph->opac.synchrotron_bridge || ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return 0; }
+  if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return 0; } /* sim.zig:335 */
   if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return 0; }
   kzo_sim *s = (kzo_sim *)calloc(1, sizeof(kzo_sim));
   s->grid = *gr;
@@ -133,6 +144,12 @@ kzo_sim *kzo_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g
   long ncorn = (long)(gr->nx + 1) * (gr->ny + 1) * (gr->nz + 1);
   for (int c = 0; c < 3; ++c) s->emf[c] = (double *)calloc(ncorn, sizeof(double));
   s->tstepdenmin = 1.0e50;
+  s->rijvisc = (double *)calloc((size_t)s->ncell * 16, sizeof(double));
+  s->dyn_a = (double *)calloc((size_t)s->ncell, sizeof(double));
+  s->vecpot = (double *)calloc((size_t)s->ncell * 6, sizeof(double));
+  s->scaleth = (double *)calloc((size_t)gr->nx, sizeof(double));
+  s->rhor = kzo_r_horizon_bl(ph->bh_spin);
+  s->risco = kzo_r_isco_bl(ph->bh_spin);
   /* set_grid min-size scan incl. its mdx quirk (sim.zig:447-466) */
   double mdx = -1, mdy = -1, mdz = -1;
   for (long ix = 0; ix < gr->nx; ++ix)
@@ -153,6 +170,7 @@ void kzo_sim_destroy(kzo_sim *s) {
   for (unsigned i = 0; i < sizeof(fl) / sizeof(fl[0]); ++i) free(fl[i]);
   for (int d = 0; d < 3; ++d) { free(s->pb_l[d]); free(s->pb_r[d]); free(s->fl_l[d]); free(s->fl_r[d]); free(s->flb[d]); }
   free(s->scal); free(s->flags);
+  free(s->rijvisc); free(s->dyn_a); free(s->vecpot); free(s->scaleth);
   for (int c = 0; c < 3; ++c) free(s->emf[c]);
   free(s);
 }
@@ -565,6 +583,392 @@ static void metric_source(kzo_sim *s, long ix, long iy, long iz, double ss[NV]) 
       }
 }
 
+/* ---- BL sidecar (metric/precompute.zig:361-380, :457-475) ---- */
+void kzo_sim_set_bl_tables(kzo_sim *s, const double *bl) { s->bl = bl; }
+static const double *bl_rec(const kzo_sim *s, long ix, long iy, long iz) { return s->bl + cidx(s, ix, iy, iz) * BL_REC; }
+static void bl_geo(const kzo_sim *s, long ix, long iy, long iz, kzo_geo *g) {
+  const double *r = bl_rec(s, ix, iy, iz);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) { g->g[i][j] = r[i * 4 + j]; g->G[i][j] = r[16 + i * 4 + j]; }
+  g->alpha = r[32];
+  g->gdet = r[33];
+  g->gttpert = 0;
+  g->x1 = r[34];
+}
+static int is_corner(const kzo_sim *s, long ix, long iy, long iz) {  /* sim.zig:660-666 */
+  int n = 0;
+  if (ix < 0 || ix >= s->grid.nx) n += 1;
+  if (iy < 0 || iy >= s->grid.ny) n += 1;
+  if (iz < 0 || iz >= s->grid.nz) n += 1;
+  return n >= 2;
+}
+static double ccoord(const kzo_sim *s, int idim, long i) {
+  return idim == 1 ? xc_(s, i) : (idim == 2 ? yc_(s, i) : zc_(s, i));
+}
+/* math/misc.zig:9-12 stepFunction */
+static double step_function(double x, double x9) {
+  double k = 1.47222 / x9;
+  return 1.0 / (1.0 + kzm_exp(-2.0 * k * x));
+}
+/* relele.convertBoth(VELR) for the velocity slots starting at `base` */
+static void conv_both_velr(const double *pp, int base, const kzo_geo *g, double con[4], double cov[4]) {
+  double v[4] = {0.0, pp[base], pp[base + 1], pp[base + 2]};
+  kzo_velr_to_vel4(v, g, con);
+  kzo_lower(con, g, cov);
+}
+
+/* ---- radiative shear viscosity (sim/rijvisc.zig, physics/radvisc.zig) ---- */
+/* radvisc.zig:72-131 shearFromGradients: sigma_ij from the assembled gradients */
+static void shear_from_gradients(const double du[4][4], const double du2[4][4], const double ucon[4],
+                                 const double ucov[4], const kzo_geo *gm, const double *kr, double sig[4][4]) {
+  double dcu[4][4], dcudiag[4];
+  for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j < 4; ++j) {
+      double krsum = 0;
+      for (int k = 0; k < 4; ++k) krsum += kr[k * 16 + i * 4 + j] * ucov[k];
+      dcu[i][j] = du[i][j] - krsum;
+    }
+    double krsum = 0;
+    for (int k = 0; k < 4; ++k) krsum += kr[i * 16 + i * 4 + k] * ucon[k];
+    dcudiag[i] = du2[i][i] + krsum;
+  }
+  double theta = 0;
+  for (int i = 0; i < 4; ++i) theta += dcudiag[i];
+  double p11[4][4], p21[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      p11[i][j] = gm->g[i][j] + ucov[i] * ucov[j];
+      p21[i][j] = (i == j ? 1.0 : 0.0) + ucon[i] * ucov[j];
+    }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) sig[i][j] = 0.0;
+  for (int i = 1; i < 4; ++i)
+    for (int j = 1; j < 4; ++j) {
+      double sum1 = 0, sum2 = 0;
+      for (int k = 0; k < 4; ++k) {
+        sum1 += dcu[i][k] * p21[k][j];
+        sum2 += dcu[j][k] * p21[k][i];
+      }
+      sig[i][j] = 0.5 * (sum1 + sum2) - 1.0 / 3.0 * theta * p11[i][j];
+    }
+  for (int i = 1; i < 4; ++i) {
+    double v = -1.0 / ucon[0] * (ucon[1] * sig[1][i] + ucon[2] * sig[2][i] + ucon[3] * sig[3][i]);
+    sig[i][0] = v;
+    sig[0][i] = v;
+  }
+  sig[0][0] = -1.0 / ucon[0] * (ucon[1] * sig[1][0] + ucon[2] * sig[2][0] + ucon[3] * sig[3][0]);
+}
+
+/* rijvisc.zig:46-172 calcShearLab (radiation-frame velocities FX..FZ) */
+static void calc_shear_lab(const kzo_sim *s, long ix, long iy, long iz, double sig[4][4]) {
+  const double *pp0 = s->p + cidx(s, ix, iy, iz) * NV;
+  kzo_geo geom;
+  geo_c(s, ix, iy, iz, &geom);
+  double du[4][4] = {{0}}, du2[4][4] = {{0}};
+  double ucon[4], ucov[4];
+  conv_both_velr(pp0, FX, &geom, ucon, ucov);
+  long nd[3] = {s->grid.nx, s->grid.ny, s->grid.nz};
+  long nx = nd[0], ny = nd[1], nz = nd[2];
+  for (int idim = 1; idim < 4; ++idim) {
+    int dim = idim - 1;
+    if (nd[dim] <= 1) continue;
+    long cm[3] = {ix, iy, iz}, cp[3] = {ix, iy, iz};
+    cm[dim] -= 1;
+    cp[dim] += 1;
+    const double *ppm1 = s->p + cidx(s, cm[0], cm[1], cm[2]) * NV;
+    const double *ppp1 = s->p + cidx(s, cp[0], cp[1], cp[2]) * NV;
+    kzo_geo gm, gp;
+    geo_c(s, cm[0], cm[1], cm[2], &gm);
+    geo_c(s, cp[0], cp[1], cp[2], &gp);
+    double umc[4], umv[4], upc[4], upv[4];
+    conv_both_velr(ppm1, FX, &gm, umc, umv);
+    conv_both_velr(ppp1, FX, &gp, upc, upv);
+    double xm = ccoord(s, idim, cm[dim]), xc = ccoord(s, idim, dim == 0 ? ix : (dim == 1 ? iy : iz)),
+           xp = ccoord(s, idim, cp[dim]);
+    for (int i = 0; i < 4; ++i) {
+      double dc = (upv[i] - umv[i]) / (xp - xm);
+      double dr = (upv[i] - ucov[i]) / (xp - xc);
+      double dl = (ucov[i] - umv[i]) / (xc - xm);
+      double dc2 = (upc[i] - umc[i]) / (xp - xm);
+      double dr2 = (upc[i] - ucon[i]) / (xp - xc);
+      double dl2 = (ucon[i] - umc[i]) / (xc - xm);
+      if ((ix < 0 && iy == 0 && iz == 0 && idim != 1) || (iy < 0 && ix == 0 && iz == 0 && idim != 2) ||
+          (iz < 0 && ix == 0 && iy == 0 && idim != 3)) {
+        du[i][idim] = dr; du2[i][idim] = dr2;
+      } else if ((ix < 0 && iy == ny - 1 && iz == nz - 1 && idim != 1) ||
+                 (iy < 0 && ix == nx - 1 && iz == nz - 1 && idim != 2) ||
+                 (iz < 0 && ix == nx - 1 && iy == ny - 1 && idim != 3)) {
+        du[i][idim] = dl; du2[i][idim] = dl2;
+      } else if ((ix >= nx && iy == 0 && iz == 0 && idim != 1) || (iy >= ny && ix == 0 && iz == 0 && idim != 2) ||
+                 (iz >= nz && ix == 0 && iy == 0 && idim != 3)) {
+        du[i][idim] = dr; du2[i][idim] = dr2;
+      } else if ((ix >= nx && iy == ny - 1 && iz == nz - 1 && idim != 1) ||
+                 (iy >= ny && ix == nx - 1 && iz == nz - 1 && idim != 2) ||
+                 (iz >= nz && ix == nx - 1 && iy == ny - 1 && idim != 3)) {
+        du[i][idim] = dl; du2[i][idim] = dl2;
+      } else {
+        du[i][idim] = dc; du2[i][idim] = dc2;
+      }
+    }
+  }
+  shear_from_gradients((const double(*)[4])du, (const double(*)[4])du2, ucon, ucov, &geom,
+                       s->kris + cidx(s, ix, iy, iz) * 64, sig);
+}
+
+/* radvisc.zig:154-175 viscCoeff */
+static double visc_coeff(double chi, double mindx, double r_bl, double rhor, double alpha, double global_dt) {
+  double mfp = 1.0 / chi;
+  double rmin = 1.2 * rhor;
+  double mfplim = r_bl;
+  if (mfp > mfplim || chi < 1.0e-80) mfp = mfplim;
+  if (mfp < 0 || !kzm_isfinite(mfp)) mfp = 0;
+  mfp *= step_function(r_bl - rmin, 0.2 * rmin);
+  if (r_bl <= 1.0 * rhor) mfp = 0;
+  double nu = alpha * mfp;
+  double nulimit = mindx * mindx / 2.0 / global_dt / 2.0;
+  if (nu > nulimit) nu = nulimit;
+  return nu;
+}
+
+/* rijvisc.zig:175-268 calcRadViscCoeff + calcRijVisc: R^i_j viscous */
+static void calc_rij_visc(const kzo_sim *s, long ix, long iy, long iz, double global_dt, double out[16]) {
+  const double *pp = s->p + cidx(s, ix, iy, iz) * NV;
+  kzo_geo geom;
+  geo_c(s, ix, iy, iz, &geom);
+  double sl[4][4], su[4][4] = {{0}};
+  calc_shear_lab(s, ix, iy, iz, sl);
+  /* relele.zig:115-128 raiseBoth */
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k)
+        for (int l = 0; l < 4; ++l) su[i][j] += sl[k][l] * geom.G[i][k] * geom.G[j][l];
+  double chi = kzo_calc_chi_slim(pp, &geom, s->ph.gam, &s->ph.opac);
+  double dx[3] = {csize(s, ix, 0) * kzm_sqrt(geom.g[1][1]), csize(s, iy, 1) * kzm_sqrt(geom.g[2][2]),
+                  csize(s, iz, 2) * kzm_sqrt(geom.g[3][3])};
+  long ny = s->grid.ny, nz = s->grid.nz;
+  double mindx = (ny == 1 && nz == 1) ? dx[0]
+                 : (nz == 1)          ? kzm_fmin(dx[0], dx[1])
+                 : (ny == 1)          ? kzm_fmin(dx[0], dx[2])
+                                      : kzm_fmin(dx[0], kzm_fmin(dx[1], dx[2]));
+  double nu = visc_coeff(chi, mindx, bl_rec(s, ix, iy, iz)[34], s->rhor, s->ph.radvisc.alpha, global_dt);
+  double rv[4][4], lo[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) rv[i][j] = -2.0 * nu * pp[EE] * su[i][j];
+  kzo_lower_second((const double(*)[4])rv, &geom, lo);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out[i * 4 + j] = lo[i][j];
+}
+
+/* rijvisc.zig:266-311 calcRijViscTotal: domain + one-cell ring, corners skipped */
+static void calc_rij_visc_total(kzo_sim *s, double global_dt) {
+  memset(s->rijvisc, 0, (size_t)s->ncell * 16 * sizeof(double));
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  long xl = nx > 1 ? 1 : 0, yl = ny > 1 ? 1 : 0, zl = nz > 1 ? 1 : 0;
+  for (long iz = -zl; iz < nz + zl; ++iz)
+    for (long iy = -yl; iy < ny + yl; ++iy)
+      for (long ix = -xl; ix < nx + xl; ++ix) {
+        if (is_corner(s, ix, iy, iz)) continue;
+        calc_rij_visc(s, ix, iy, iz, global_dt, s->rijvisc + cidx(s, ix, iy, iz) * 16);
+      }
+}
+
+/* rijvisc.zig:313-353 faceAvg + addRadViscFlux (radvisc.zig:201-244): the
+ * face-averaged, velocity-damped viscous flux added to the M1 rows */
+static void add_rad_visc_flux(const kzo_sim *s, int dim, const long fc[3], const double *pp, const kzo_geo *g,
+                              double *ff) {
+  const double *a = s->rijvisc + cidx(s, fc[0], fc[1], fc[2]) * 16;
+  long c[3] = {fc[0], fc[1], fc[2]};
+  c[dim] -= 1;
+  const double *b = s->rijvisc + cidx(s, c[0], c[1], c[2]) * 16;
+  double rv[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) rv[i][j] = 0.5 * (a[i * 4 + j] + b[i * 4 + j]);
+  double uu[NV];
+  kzo_p2u(pp, uu, g, s->ph.gam);
+  int y_active = s->grid.ny > 1, z_active = s->grid.nz > 1;
+  double gdetu = g->gdet;
+  double vel[3] = {0, 0, 0};
+  for (int id = 0; id < 3; ++id)
+    for (int ii = 1; ii < 4; ++ii) {
+      if (ii == 2 && !y_active) continue;
+      if (ii == 3 && !z_active) continue;
+      if (kzm_fabs(uu[EE + ii]) < 1.0e-10 * kzm_fabs(uu[EE])) continue;
+      vel[id] = rv[id + 1][ii] / (uu[EE + ii] / gdetu) * kzm_sqrt(g->g[id + 1][id + 1]);
+    }
+  double maxv = kzm_fabs(vel[dim]);
+  double dampfac = 1.0;
+  if (maxv > s->ph.radvisc.maxvel) dampfac = s->ph.radvisc.maxvel / maxv;
+  for (int nu = 0; nu < 4; ++nu) ff[EE + nu] += gdetu * dampfac * rv[dim + 1][nu];
+}
+
+/* ---- mean-field dynamo (sim/dynamo.zig) ---- */
+static const double KZO_PI = 3.14159265358979323846;
+
+/* dynamo.zig:226-256 calcScaleHeight (single rank): density-weighted RMS of
+ * pi/2 - theta per radial column; column 0 keeps the raw sum (the reference's
+ * gix > 0 quirk) */
+static void calc_scale_height(kzo_sim *s) {
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  for (long ix = 0; ix < nx; ++ix) {
+    double sigma = 0, sth = 0;
+    for (long iy = 0; iy < ny; ++iy)
+      for (long iz = 0; iz < nz; ++iz) {
+        double gd = s->g[cidx(s, ix, iy, iz) * 20 + 19];
+        double th = bl_rec(s, ix, iy, iz)[35];
+        double rho = s->p[cidx(s, ix, iy, iz) * NV + RHO];
+        double dth = KZO_PI / 2.0 - th;
+        sigma += rho * gd;
+        sth += rho * gd * dth * dth;
+      }
+    s->scaleth[ix] = ix > 0 ? kzm_sqrt(sth / sigma) : sth;
+  }
+}
+
+/* dynamo.zig:67-73 dampBphi: the DAMPBETA increment of B^phi */
+static double damp_bphi(double alphabeta, double facradius, double faczh, double dt_over_pk, double beta,
+                        double betasat, double bphi) {
+  double dbphi = -alphabeta * facradius * faczh * dt_over_pk * kzm_fmax(0.0, beta - betasat) / betasat * bphi;
+  if ((dbphi + bphi) * bphi < 0.0) dbphi = -bphi;
+  return dbphi;
+}
+
+/* dynamo.zig:104-151 dynamoDeltaA: returns 0 and leaves *bphi_out = bphi
+ * when r < 1.0001 r_horizon */
+static double dynamo_delta_a(const kz_dynamo_params *dp, double a, double dt, double rhor, double risco, double r,
+                             double th, double bphi, double bsq, double angle, double prermhd, double gg33,
+                             double scaleth, double *bphi_out) {
+  *bphi_out = bphi;
+  if (r < 1.0001 * rhor) return 0.0;
+  double omk = 1.0 / (a + kzm_sqrt(r * r * r));
+  double pk = 2.0 * KZO_PI / omk;
+  double facangle = 0;
+  if (kzm_isfinite(angle)) {
+    if (angle < -1.0) angle = -1.0;
+    facangle = kzm_fmax(0.0, (dp->thetaangle - angle) / dp->thetaangle);
+  }
+  double facradius = step_function(r - 1.0 * risco, 0.1 * risco);
+  double beta = 0.5 * bsq / prermhd;
+  double hrdtheta;
+  if (dp->calchronthego) {
+    hrdtheta = scaleth;
+    if (hrdtheta > 0.9 * KZO_PI / 2.0) hrdtheta = 0.9 * KZO_PI / 2.0;
+  } else {
+    hrdtheta = dp->expectedhr * KZO_PI / 2.0;
+  }
+  double zh = (KZO_PI / 2.0 - th) / hrdtheta;
+  double faczh = kzm_fmax(0.0, 1.0 - zh * zh);
+  double facmagnetization = faczh;
+  double effalpha = dp->alphadynamo;
+  if (dp->alphaflipssign) effalpha = -(KZO_PI / 2.0 - th) / (hrdtheta / 2.0) * dp->alphadynamo;
+  double aphi = effalpha * (hrdtheta / (KZO_PI / 2.0)) / 0.4 * dt / pk * r * gg33 * bphi * facradius *
+                facmagnetization * facangle;
+  if (dp->dampbeta) *bphi_out = bphi + damp_bphi(dp->alphabeta, facradius, faczh, dt / pk, beta, dp->betasaturated, bphi);
+  return aphi;
+}
+
+/* dynamo.zig:335-368 fieldAngle: -b^r b^phi sqrt(g_rr g_phph)/b^2 in BL */
+static int field_angle(const double *pp, const kzo_geo *gm, const kzo_geo *gbl, const double *J, double *angle,
+                       double *bsq) {
+  double ppbl[NV];
+  memcpy(ppbl, pp, sizeof ppbl);
+  int e = trans_pmhd(ppbl, gm, gbl, J);
+  if (e) return e;
+  double ucon[4], ucov[4];
+  kzo_ucon_ucov_from_prims(ppbl, gbl, ucon, ucov);
+  double B[3] = {ppbl[B1], ppbl[B2], ppbl[B3]}, bcon[4], bcov[4], b2;
+  kzo_bcon_bcov_bsq(B, ucon, ucov, gbl, bcon, bcov, &b2);
+  double brbphi = kzm_sqrt(gbl->g[1][1] * gbl->g[3][3]) * bcon[1] * bcon[3];
+  *angle = -brbphi / b2;
+  *bsq = b2;
+  return KZO_OK;
+}
+
+/* dynamo.zig:370-460 mimicDynamo: Delta A_phi over the domain + ring
+ * (corners included), DAMPBETA on p[B3], curl, superpose on the domain */
+static int mimic_dynamo(kzo_sim *s, double dt) {
+  long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  long xl = nx > 1 ? 1 : 0, yl = ny > 1 ? 1 : 0, zl = nz > 1 ? 1 : 0;
+  memset(s->dyn_a, 0, (size_t)s->ncell * sizeof(double));
+  const kz_dynamo_params *dp = &s->ph.dynamo_params;
+  for (long iz = -zl; iz < nz + zl; ++iz)
+    for (long iy = -yl; iy < ny + yl; ++iy)
+      for (long ix = -xl; ix < nx + xl; ++ix) {
+        kzo_geo geom, gbl;
+        geo_c(s, ix, iy, iz, &geom);
+        bl_geo(s, ix, iy, iz, &gbl);
+        const double *rec = bl_rec(s, ix, iy, iz);
+        double *pp = s->p + cidx(s, ix, iy, iz) * NV;
+        double angle, bsq;
+        int e = field_angle(pp, &geom, &gbl, rec + 36, &angle, &bsq);
+        if (e) return e;
+        double r = rec[34], th = rec[35];
+        if (r < 1.0001 * s->rhor) continue;
+        double prermhd = (s->ph.gam - 1.0) * pp[UU];
+        double ehat = -kzo_calc_ff_rtt(pp, &geom, 0);
+        prermhd += ehat / 3.0;
+        long gix = ix < 0 ? 0 : (ix > nx - 1 ? nx - 1 : ix);
+        double bphi_new;
+        double aphi = dynamo_delta_a(dp, s->ph.bh_spin, dt, s->rhor, s->risco, r, th, pp[B3], bsq, angle, prermhd,
+                                     geom.g[3][3], s->scaleth[gix], &bphi_new);
+        s->dyn_a[cidx(s, ix, iy, iz)] = aphi;
+        if (dp->dampbeta) pp[B3] = bphi_new;
+      }
+  /* ct.zig:228-264 cornerAverageA (2D) of (0, 0, Delta A_phi) */
+  for (long iy = 0; iy <= ny; ++iy)
+    for (long ix = 0; ix <= nx; ++ix) {
+      double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
+      v[0] = 1.0 / 4.0 * (0.0 + 0.0 + 0.0 + 0.0);
+      v[1] = v[0];
+      v[2] = 1.0 / 4.0 * (s->dyn_a[cidx(s, ix, iy, 0)] + s->dyn_a[cidx(s, ix, iy - 1, 0)] +
+                          s->dyn_a[cidx(s, ix - 1, iy, 0)] + s->dyn_a[cidx(s, ix - 1, iy - 1, 0)]);
+    }
+  /* ct.zig:279-331 calcBfromACore (2D) then dynamo.zig:432-460 superpose */
+  for (long iy = 0; iy < ny; ++iy)
+    for (long ix = 0; ix < nx; ++ix) {
+      double dA[4][4] = {{0}};
+      for (int c = 0; c < 3; ++c) {
+        const double a00 = s->vecpot[cidx(s, ix, iy, 0) * 6 + c], a01 = s->vecpot[cidx(s, ix, iy + 1, 0) * 6 + c];
+        const double a10 = s->vecpot[cidx(s, ix + 1, iy, 0) * 6 + c],
+                     a11 = s->vecpot[cidx(s, ix + 1, iy + 1, 0) * 6 + c];
+        dA[c + 1][2] = -(a00 - a01 + a10 - a11) / (2.0 * csize(s, iy, 1));
+        dA[c + 1][1] = -(a00 + a01 - a10 - a11) / (2.0 * csize(s, ix, 0));
+        dA[c + 1][3] = 0;
+      }
+      double gdet = s->g[cidx(s, ix, iy, 0) * 20 + 19];
+      double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
+      v[3] = (dA[2][3] - dA[3][2]) / gdet;
+      v[4] = (dA[3][1] - dA[1][3]) / gdet;
+      v[5] = (dA[1][2] - dA[2][1]) / gdet;
+    }
+  for (long iy = 0; iy < ny; ++iy)
+    for (long ix = 0; ix < nx; ++ix) {
+      double *pp = s->p + cidx(s, ix, iy, 0) * NV;
+      const double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
+      pp[B1] += v[3];
+      pp[B2] += v[4];
+      pp[B3] += v[5];
+      kzo_geo geom;
+      geo_c(s, ix, iy, 0, &geom);
+      double uu[NV];
+      kzo_p2u(pp, uu, &geom, s->ph.gam);
+      double *u = s->u + cidx(s, ix, iy, 0) * NV;
+      u[B1] = uu[B1];
+      u[B2] = uu[B2];
+      u[B3] = uu[B3];
+    }
+  return KZO_OK;
+}
+
+/* dynamo.zig:462-469 applyDynamo */
+static int apply_dynamo(kzo_sim *s, double dt) {
+  calc_scale_height(s);
+  int e = kzo_sim_set_bc(s, 0);
+  if (e) return e;
+  e = mimic_dynamo(s, dt);
+  if (e) return e;
+  return kzo_sim_calc_u2p(s);
+}
+
 /* ---- explicit operator (sim.zig:1209-1582) ---- */
 static long cross_lo(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? -1 : 0; }
 static long cross_hi(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? ndim(s, dim) + 1 : ndim(s, dim); }
@@ -592,6 +996,7 @@ static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
     kzo_check_floors_mhd(pl, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
     int e = kzo_f_flux_prime(pl, dim, &g, s->ph.gam, ffl);
     if (e) return e;
+    if (s->ph.radviscosity) add_rad_visc_flux(s, dim, cellc, pl, &g, ffl); /* sim.zig:1349-1352 */
   }
   long cp[3] = {cellc[0], cellc[1], cellc[2]};
   cp[dim] = i + 1;
@@ -601,6 +1006,7 @@ static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
     kzo_check_floors_mhd(pr, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
     int e = kzo_f_flux_prime(pr, dim, &g, s->ph.gam, ffr);
     if (e) return e;
+    if (s->ph.radviscosity) add_rad_visc_flux(s, dim, cp, pr, &g, ffr); /* sim.zig:1361-1364 */
   }
   memcpy(face(s->pb_r[dim], s, dim, cellc[0], cellc[1], cellc[2]), pl, sizeof pl);
   if (dol) memcpy(face(s->fl_r[dim], s, dim, cellc[0], cellc[1], cellc[2]), ffl, sizeof ffl);
@@ -863,6 +1269,8 @@ int kzo_sim_step(kzo_sim *s, const double *forced_dt) {
   s->tstepdenmin = 1.0e50;
   const double g = GAM_IMEX;
   int e;
+  if ((s->ph.radviscosity || s->ph.dynamo) && !s->bl) return KZ_E_INVALID_CONFIG;
+  if (s->ph.radviscosity) calc_rij_visc_total(s, dt); /* sim.zig:1819-1823 */
   save_timesteps(s);
   copy_full(s, s->ut0, s->u);
   if ((e = kzo_sim_op_implicit(s, dt * g))) return e;
@@ -873,6 +1281,7 @@ int kzo_sim_step(kzo_sim *s, const double *forced_dt) {
   kzo_sim_do_correct(s);
   if ((e = kzo_sim_set_bc(s, 0))) return e;
   if ((e = kzo_sim_op_explicit(s, dt))) return e;
+  if (s->ph.dynamo && (e = apply_dynamo(s, dt))) return e; /* sim.zig:1845-1849 */
   copy_entropy_count(s);
   stage_deriv(s, s->dut1, s->u, s->ut1, dt);
 
@@ -889,6 +1298,7 @@ int kzo_sim_step(kzo_sim *s, const double *forced_dt) {
   kzo_sim_do_correct(s);
   if ((e = kzo_sim_set_bc(s, 0))) return e;
   if ((e = kzo_sim_op_explicit(s, dt))) return e;
+  if (s->ph.dynamo && (e = apply_dynamo(s, dt))) return e; /* sim.zig:1872-1876 */
   stage_deriv(s, s->dut2, s->u, s->ut2, dt);
 
   stage_combine(s, s->u, s->ut0, dt / 2.0, s->dut1, dt / 2.0, s->dut2);
@@ -932,4 +1342,50 @@ double kzo_sim_postinit(kzo_sim *s, double maxbeta) {
       p2u_cell(s, ix, iy, 0);
     }
   return fac;
+}
+
+/* test hooks: the per-step radiative-viscosity pass and one dynamo application */
+int kzo_sim_calc_rij_visc_total(kzo_sim *s, double global_dt) {
+  if (!s->bl) return KZ_E_INVALID_CONFIG;
+  calc_rij_visc_total(s, global_dt);
+  return KZO_OK;
+}
+void kzo_sim_get_rijvisc(kzo_sim *s, double *out) { memcpy(out, s->rijvisc, (size_t)s->ncell * 16 * sizeof(double)); }
+int kzo_sim_apply_dynamo(kzo_sim *s, double dt) {
+  if (!s->bl) return KZ_E_INVALID_CONFIG;
+  return apply_dynamo(s, dt);
+}
+void kzo_sim_get_scaleth(kzo_sim *s, double *out) { memcpy(out, s->scaleth, (size_t)s->grid.nx * sizeof(double)); }
+double kzo_step_function(double x, double x9) { return step_function(x, x9); }
+double kzo_visc_coeff(double chi, double mindx, double r_bl, double rhor, double alpha, double global_dt) {
+  return visc_coeff(chi, mindx, r_bl, rhor, alpha, global_dt);
+}
+void kzo_shear_from_gradients(const double *du16, const double *du2_16, const double *ucon, const double *ucov,
+                              const kz_geom *g, const double *kr64, double *sig16) {
+  kzo_geo gm;
+  kzo_geo_from_kz(g, &gm);
+  shear_from_gradients((const double(*)[4])du16, (const double(*)[4])du2_16, ucon, ucov, &gm, kr64,
+                       (double(*)[4])sig16);
+}
+double kzo_dynamo_delta_a(const kz_dynamo_params *dp, double a, double dt, double rhor, double risco, double r,
+                          double th, double bphi, double bsq, double angle, double prermhd, double gg33,
+                          double scaleth, double *bphi_out) {
+  return dynamo_delta_a(dp, a, dt, rhor, risco, r, th, bphi, bsq, angle, prermhd, gg33, scaleth, bphi_out);
+}
+double kzo_damp_bphi(double alphabeta, double facradius, double faczh, double dt_over_pk, double beta,
+                     double betasat, double bphi) {
+  return damp_bphi(alphabeta, facradius, faczh, dt_over_pk, beta, betasat, bphi);
+}
+int kzo_sim_shear_lab(kzo_sim *s, long ix, long iy, long iz, double *sig16) {
+  calc_shear_lab(s, ix, iy, iz, (double(*)[4])sig16);
+  return KZO_OK;
+}
+void kzo_sim_get_dyn_a(kzo_sim *s, double *out) { memcpy(out, s->dyn_a, (size_t)s->ncell * sizeof(double)); }
+/* the dynamo core without the final calcU2p: scale height, setBc, mimicDynamo */
+int kzo_sim_mimic_dynamo(kzo_sim *s, double dt) {
+  if (!s->bl) return KZ_E_INVALID_CONFIG;
+  calc_scale_height(s);
+  int e = kzo_sim_set_bc(s, 0);
+  if (e) return e;
+  return mimic_dynamo(s, dt);
 }

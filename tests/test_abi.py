@@ -59,3 +59,27 @@ def test_create_rejects_3d():
     tab = host.metric_tables(grid)
     with pytest.raises(RuntimeError, match="Unsupported|-7"):
         hip.Sim(grid, T.physics_puffy(), tab)
+
+
+def test_metric_tables_symmetric_bitwise():
+    """The device keeps g and the Christoffels as upper triangles; MetricCache
+    (and the host library restating it) builds them symmetric bit for bit."""
+    import numpy as np
+    for nx, ny in ((32, 30), (96, 90)):
+        grid = T.grid_puffy(nx, ny)
+        t = host.metric_tables(grid)
+        kr = t["kris"].reshape(-1, 4, 4, 4).view(np.uint64)
+        assert np.array_equal(kr, kr.transpose(0, 1, 3, 2))
+        for k in ("g", "gb0", "gb1", "gb2"):
+            a = t[k].reshape(-1, 4, 5)[:, :, :4].view(np.uint64)
+            assert np.array_equal(a, a.transpose(0, 2, 1)), k
+
+
+def test_create_rejects_asymmetric_metric():
+    """An asymmetric block is rejected before any device work."""
+    grid = T.grid_puffy(16, 12)
+    tab = host.metric_tables(grid)
+    g = tab["g"].copy()
+    g[20 * 50 + 1] = g[20 * 50 + 5] * (1 + 1e-15) + 1e-300  # g[0][1] != g[1][0] in cell 50
+    with pytest.raises(RuntimeError, match="InvalidConfig"):
+        hip.Sim(grid, T.physics_puffy(), dict(tab, g=g))

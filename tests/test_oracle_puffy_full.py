@@ -1,0 +1,126 @@
+"""The oracle's whole PUFFY step with radiative viscosity and the dynamo on.
+
+Two reference anchors for the full production option set
+(koral/testing/selfscenarios.zig:71-90: polar-axis correction, radiative
+viscosity, dynamo, PUFFY BCs):
+
+* the C forced-dt step golden step/puffy64.kstp (64x60), driven the way the
+  reference's own test drives it (puffystep_golden_tests.zig:166-300): C's
+  post-init state loaded bit-for-bit, ghosts rebuilt, C's dt sequence forced;
+  gated by that test's own lines — flag maps exact, >= 95% of cells within
+  1e-3 field-scale, no cell above 0.15, the first step within 1e-4;
+* the Zig self-golden selfgolden/puffy_fast.kslf (32x30, 3 CFL steps from
+  the Zig init).  The oracle's init reproduces Zig's to ~1e-10 (libm), so the
+  gate is the flag map, the CFL clock and a field-scale bound rather than
+  bits.
+"""
+import numpy as np
+import pytest
+
+import kzo
+from goldens import read_kslf, read_kstp
+from koral_zig_amd import abi_types as T
+from koral_zig_amd import host
+
+
+# kstp flag columns (golden.zig:62-160): entropy, hd_fixup, rad_fixup, radimp_fixup
+KSTP_FLAGS = [T.FLAG_ENTROPY, T.FLAG_HD_FIXUP, T.FLAG_RAD_FIXUP, T.FLAG_RADIMP_FIXUP]
+
+
+def _full_phys():
+    return T.physics_puffy(radviscosity=True, dynamo=True)
+
+
+def _r_bl_col(grid):
+    """Boyer-Lindquist radius of each domain column (MKS2: r = exp(x1) + r0)."""
+    bl = host.bl_tables(grid).reshape(grid.ny + 6, grid.nx + 6, host.BL_REC)
+    return bl[3, 3:3 + grid.nx, 34]
+
+
+@pytest.mark.slow
+def test_puffy64_forced_dt_vs_c():
+    k = read_kstp("step/puffy64.kstp.gz")
+    grid = T.grid_puffy(k["nx"], k["ny"])
+    phys = _full_phys()
+    tab = host.metric_tables(grid)
+    sim = kzo.Sim(grid, phys, tab)
+    sim.set_bl_tables(host.bl_tables(grid))
+    p, u, _ = sim.get_state()
+    r0 = k["recs"][0]
+    ng = 3
+    dom = (slice(None), slice(ng, ng + grid.ny), slice(ng, ng + grid.nx))
+    p[dom] = r0["p"]
+    u[dom] = r0["u"]
+    sim.set_state(p=p, u=u)
+    sim.set_bc(True)
+    sim.init_timestep_guess()
+    rbl = _r_bl_col(grid)
+    nc = 2
+    report = []
+    for i, r in enumerate(k["recs"][1:], 1):
+        sim.step(r["dt"])
+        assert abs(sim.counters()["t"] - r["t"]) <= 1e-11 * abs(r["t"])
+        p, _, f = sim.get_state()
+        zp = p[dom]
+        scale = np.maximum(np.abs(r["p"]).max(axis=(0, 1, 2)), 1e-300)
+        cf = r["flags"].astype(np.int32)
+        zf = f[dom][..., KSTP_FLAGS[:k["nflags"]]]
+        flags_ok = (cf == zf).all(axis=-1)
+        dev_v = np.abs(zp - r["p"]) / scale
+        # B1..B3: the induction and the dynamo's A_phi superposition are
+        # well-conditioned, so the field tracks C to ~1e-12 (with the dynamo
+        # switched off it is ~5e-9 off by the third step)
+        bdev = float(dev_v[..., 6:9].max())
+        assert bdev <= 1e-10, (i, bdev)
+        dev = dev_v.max(axis=-1)
+        dev = np.where(flags_ok, dev, 0.0)
+        over = (dev > 1e-3).mean()
+        iy = np.arange(grid.ny)[None, :, None]
+        interior = (rbl[None, None, :] >= 2.5) & (iy > nc) & (iy < grid.ny - 1 - nc)
+        report.append((i, float(dev.max()), float(np.where(interior, dev, 0).max()), float(over),
+                       float(flags_ok.mean())))
+    print("puffy64 step, max dev, interior, cells>1e-3, flags:", report)
+    for i, mx, inner, over, fl in report:
+        assert fl == 1.0, report
+        assert over <= 0.05, report
+        assert mx <= 0.15, report
+    assert report[0][1] <= 1e-4, report
+
+
+@pytest.mark.parametrize("name", ["puffy_fast.kslf", "puffy_full.kslf"])
+def test_selfgolden_from_zig_init(name):
+    """Zig's own post-init snapshot (ghosts included) stepped with CFL dt and
+    compared with Zig's final snapshot (selfscenarios.zig:92-121): the CFL
+    clock to 1e-14, the flag map exactly, B1..B3 to 1e-11 (the dynamo off
+    would leave them ~5e-9 off), the other non-radiative slots to 1e-8, and
+    the radiation velocities FX..FZ to 2e-2 field-scale.  That last bound is
+    the oracle's own sensitivity: a 1e-14 perturbation of the post-init
+    hydro state moves FX by 3e-3 field-scale within three steps (the stiff
+    optically thick torus), and switching the radiative viscosity off moves
+    it by 0.25."""
+    s = read_kslf("selfgolden/" + name)
+    grid = T.grid_puffy(s["nx"], s["ny"])
+    sim = kzo.Sim(grid, _full_phys(), host.metric_tables(grid))
+    sim.set_bl_tables(host.bl_tables(grid))
+    p, u, _ = sim.get_state()
+    f0 = s["fields"][0]
+    sim.set_state(p=f0["p"].reshape(p.shape).copy(), u=f0["u"].reshape(u.shape).copy())
+    sim.init_timestep_guess()
+    sc = s["scalars"]
+    assert sim.counters()["tstepdenmax"] == sc[0, 3]
+    for i in range(1, sc.shape[0]):
+        sim.step()
+        c = sim.counters()
+        assert abs(c["dt"] - sc[i, 1]) <= 1e-14 * sc[i, 1], (i, c["dt"], sc[i, 1])
+        assert abs(c["t"] - sc[i, 0]) <= 1e-14 * sc[i, 0]
+    last = s["fields"][-1]
+    p, u, f = sim.get_state()
+    assert (f == last["flags"].reshape(f.shape)).all()
+    pr = last["p"].reshape(p.shape)
+    ng = 3
+    dom = (slice(None), slice(ng, ng + grid.ny), slice(ng, ng + grid.nx))
+    scale = np.maximum(np.abs(pr[dom]).max(axis=(0, 1, 2)), 1e-300)
+    dev = (np.abs(p[dom] - pr[dom]) / scale).max(axis=(0, 1, 2))
+    assert dev[6:9].max() <= 1e-11, dev
+    assert max(dev[0:6].max(), dev[9]) <= 1e-8, dev
+    assert dev[10:13].max() <= 2e-2, dev
