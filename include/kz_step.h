@@ -28,8 +28,10 @@ typedef struct kz_sim kz_sim;
  * the caller (kz_host_metric_tables) and the PUFFY outer-boundary table
  * (kz_host_bc_xhi_tables). Returns NULL and sets *err on failure:
  * KZ_E_UNSUPPORTED for nz != 1 or an option outside this build
- * (radviscosity, dynamo, reduce-order, synchrotron bridge, ZAMO floors),
- * KZ_E_INVALID_CONFIG for sim.zig:319-363's preconditions. */
+ * (reduce-order, synchrotron bridge, ZAMO floors), KZ_E_INVALID_CONFIG for
+ * sim.zig:319-363's preconditions (including radiative viscosity without
+ * opacities).  With radviscosity or dynamo set, the BL sidecar must be
+ * loaded with kz_sim_set_bl_tables before the first step. */
 kz_sim *kz_sim_create(const kz_grid *grid, const kz_physics *phys, const double *g, const double *gcon,
                       const double *kris, const double *gb0, const double *gconb0, const double *gb1,
                       const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err);
@@ -64,6 +66,20 @@ int kz_op_explicit(kz_sim *s, double dt);
 int kz_op_implicit(kz_sim *s, double dt);
 int kz_do_correct(kz_sim *s);
 
+/* The MetricCache BL sidecar (koral/metric/precompute.zig:148,379,457-470:
+ * blGeom + jacMy2Bl) as kz_host_bl_tables builds it: KZ_BL_REC doubles per
+ * padded cell (BL g[16], G[16], alpha, gdet, r, theta, J_mks2->bl[16]).
+ * KZ_E_INVALID_CONFIG unless radviscosity or dynamo is on. */
+int kz_sim_set_bl_tables(kz_sim *s, const double *bl);
+/* calcRijViscTotal (koral/sim/rijvisc.zig:266-311): the viscous R^i_j on the
+ * domain and its one-cell ring (corners 0) for the step size global_dt;
+ * kz_sim_get_rijvisc copies it out, 16 doubles per padded cell. */
+int kz_calc_rij_visc_total(kz_sim *s, double global_dt);
+int kz_sim_get_rijvisc(kz_sim *s, double *out);
+/* applyDynamo (koral/sim/dynamo.zig:462-469): scale height, setBc,
+ * mimicDynamo over dt, calcU2p. */
+int kz_apply_dynamo(kz_sim *s, double dt);
+
 /* Sim.step (koral/sim.zig:1792-1899): one RK2IMEX step; forced_dt NULL
  * takes the CFL step 1/tstepdenmax. Synchronous; returns the first error
  * any kernel of the step reported. */
@@ -90,7 +106,9 @@ enum {
   KZ_K_U2P = 4,        /* u2p + floors */
   KZ_K_IMPLICIT = 5,   /* implicit four-force Newton */
   KZ_K_BC = 6,         /* boundary conditions */
-  KZ_K_COUNT = 7
+  KZ_K_VISC = 7,       /* viscous R^i_j (radiative viscosity) */
+  KZ_K_DYNAMO = 8,     /* scale height + Delta A_phi + curl (dynamo) */
+  KZ_K_COUNT = 9
 };
 /* Time every launch of one kernel class with HIP events recorded on the
  * sim's stream (kernel = -1 turns timing off); kz_sim_kernel_time returns

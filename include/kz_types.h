@@ -12,6 +12,8 @@
 #include <stdint.h>
 
 #define KZ_NV 13
+/* doubles per padded cell of the BL sidecar (kz_host_bl_tables) */
+#define KZ_BL_REC 52
 /* primitive / conserved slot map (koral/layout.zig:114-137, config.puffy) */
 enum {
   KZ_RHO = 0, KZ_UU = 1, KZ_VX = 2, KZ_VY = 3, KZ_VZ = 4, KZ_ENTR = 5,

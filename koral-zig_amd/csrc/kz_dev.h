@@ -545,7 +545,11 @@ KZD void reconstruct(int order, double theta, const double u[5], const double dx
 KZD double laxf(double fl, double fr, double ul, double ur, double ag) { return 0.5 * (fr + fl - ag * (ur - ul)); }
 
 // ---- physics/flux.zig:40-134 fFluxPrime -------------------------------------
-KZD int flux_prime(const double *pp, int idim, const Geo &gm, double gam, double *ff) {
+// IDIM is a template parameter so that the row index d = IDIM + 1 into the
+// 4-vectors and T^mu_nu is a compile-time constant (registers, not scratch).
+template <int IDIM>
+KZD int flux_prime_d(const double *pp, const Geo &gm, double gam, double *ff) {
+  constexpr int idim = IDIM;
   double gdetu = gm.gdet;
   double rho = pp[RHO], ugas = pp[UU], s = pp[ENTR];
   double vcon[4] = {0, pp[VX], pp[VY], pp[VZ]};
@@ -586,6 +590,11 @@ KZD int flux_prime(const double *pp, int idim, const Geo &gm, double gam, double
 #pragma unroll
   for (int i = 0; i < NV; ++i) bad |= !kzm_isfinite(ff[i]);
   return bad ? E_NANFLUX : E_OK;
+}
+KZD int flux_prime(const double *pp, int idim, const Geo &gm, double gam, double *ff) {
+  if (idim == 0) return flux_prime_d<0>(pp, gm, gam, ff);
+  if (idim == 1) return flux_prime_d<1>(pp, gm, gam, ff);
+  return flux_prime_d<2>(pp, gm, gam, ff);
 }
 
 }  // namespace kzd

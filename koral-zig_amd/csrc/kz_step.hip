@@ -30,7 +30,11 @@
 
 using namespace kzd;
 
-#define GEO_NC 35  // g[16] G[16] gdet gttpert alpha
+// geometry planes: the upper triangle of g (symmetric bit for bit, checked
+// at create), all 16 entries of G (its mathematically zero entries carry
+// either sign, so it is kept whole), then gdet, gttpert, alpha
+#define GEO_NC 29
+#define KR_NC 40  // Christoffels Gamma^l_{nu k}: symmetric in (nu, k)
 #define BCX_REC 66
 
 struct GridD {
@@ -60,7 +64,19 @@ struct Dev {
   double *imp_aux;   // [IMP_NAUX][nd] reference-state scalars, see ImpAux
   int32_t *imp_slow;           // cells that need rungs past the first
   unsigned long long *imp_nslow;
+  // the u2p fixup list
+  unsigned *sync;     // [SY_NFIX] u2p fixup list length
+  int32_t *fix_list;  // domain cells flagged for a u2p fixup
+  // radiative viscosity and dynamo (allocated when enabled)
+  double *bl;       // [BL_NC][ncell] Boyer-Lindquist sidecar, see BlPlane
+  double *rijvisc;  // [16][ncell] viscous R^i_j (domain + one-cell ring, corners 0)
+  double *dyn_a;    // [ncell] Delta A_phi of the last mimicDynamo
+  double *scaleth;  // [nx] density-weighted theta scale height per column
 };
+// BL sidecar planes (MetricCache.blGeom + jacMy2Bl, precompute.zig:379,457-470)
+enum { BL_G = 0, BL_GC = 16, BL_ALPHA = 32, BL_GDET = 33, BL_R = 34, BL_TH = 35, BL_J = 36, BL_NC = 52 };
+enum { SY_NFIX = 0, SY_N = 4 };
+
 // imp_aux planes
 enum { IA_EHAT = 0, IA_UINT, IA_SGAS, IA_KES, IA_KGA, IA_KRA, IA_KRR, IA_SW, IMP_NAUX };
 
@@ -68,6 +84,7 @@ enum { SC_AHD = 0, SC_ARAD = 3, SC_TSTEPDEN = 6, SC_N = 7 };
 enum { SS_DT = 0, SS_DTG, SS_INV_DTG, SS_NINV_DTG, SS_INV_DT, SS_NINV_DT, SS_C12, SS_HALF, SS_T, SS_OWN_DT, SS_USER, SS_N };
 
 static const double GAM_IMEX = 0x1.2bec333018867p-2;  // 1 - 1/sqrt(2), correctly rounded
+static constexpr double KZ_PI = 3.14159265358979323846;
 
 // ---------------------------------------------------------------------------
 // indexing helpers
@@ -83,20 +100,28 @@ __device__ __forceinline__ double yl_(const GridD &g, long i) { return g.miny + 
 __device__ __forceinline__ double zl_(const GridD &g, long i) { return g.minz + (double)i * g.dz; }
 __device__ __forceinline__ double xc_(const GridD &g, long i) { return 0.5 * (xl_(g, i) + xl_(g, i + 1)); }
 __device__ __forceinline__ double yc_(const GridD &g, long i) { return 0.5 * (yl_(g, i) + yl_(g, i + 1)); }
+__device__ __forceinline__ double zc_(const GridD &g, long i) { return 0.5 * (zl_(g, i) + zl_(g, i + 1)); }
 __device__ __forceinline__ double csize(const GridD &g, long i, int d) {
   return d == 0 ? xl_(g, i + 1) - xl_(g, i) : (d == 1 ? yl_(g, i + 1) - yl_(g, i) : zl_(g, i + 1) - zl_(g, i));
 }
+// index of the symmetric pair (a, b), a <= b, in a 10-entry upper triangle
+__host__ __device__ constexpr int tri(int a, int b) { return a * 4 - a * (a - 1) / 2 + (b - a); }
 __device__ __forceinline__ void load_geo(const double *__restrict__ t, long stride, long i, double x1, Geo &gm) {
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      gm.g[a][b] = t[(a * 4 + b) * stride + i];
-      gm.G[a][b] = t[(16 + a * 4 + b) * stride + i];
+    for (int b = a; b < 4; ++b) {
+      double v = t[tri(a, b) * stride + i];
+      gm.g[a][b] = v;
+      gm.g[b][a] = v;
     }
-  gm.gdet = t[32 * stride + i];
-  gm.gttpert = t[33 * stride + i];
-  gm.alpha = t[34 * stride + i];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) gm.G[a][b] = t[(10 + a * 4 + b) * stride + i];
+  gm.gdet = t[26 * stride + i];
+  gm.gttpert = t[27 * stride + i];
+  gm.alpha = t[28 * stride + i];
   gm.x1 = x1;
 }
 __device__ __forceinline__ void geo_c(const GridD &g, const Dev &d, long ix, long iy, long iz, Geo &gm) {
@@ -123,6 +148,31 @@ __device__ __forceinline__ bool polar_row(const kz_physics &ph, const GridD &g, 
 }
 
 #define TID (blockIdx.x * (long)blockDim.x + threadIdx.x)
+
+// Minimum waves per SIMD each heavy kernel is compiled for (1 = no bound;
+// n bounds the kernel to 512/n vector registers).  Tuned by measurement.
+#define KZ_WPE(n) __attribute__((amdgpu_waves_per_eu(n)))
+#ifndef KZ_WPE_FACE
+#define KZ_WPE_FACE 1
+#endif
+#ifndef KZ_WPE_UPDATE
+#define KZ_WPE_UPDATE 1
+#endif
+#ifndef KZ_WPE_U2P
+#define KZ_WPE_U2P 1
+#endif
+#ifndef KZ_WPE_WS
+#define KZ_WPE_WS 1
+#endif
+#ifndef KZ_WPE_PREP
+#define KZ_WPE_PREP 1
+#endif
+#ifndef KZ_WPE_RUNG0
+#define KZ_WPE_RUNG0 1
+#endif
+#ifndef KZ_WPE_BC
+#define KZ_WPE_BC 1
+#endif
 
 // ---------------------------------------------------------------------------
 // p2u over domain cells (Sim.initCell / postinit refresh)
@@ -261,14 +311,6 @@ __device__ void bc_y_cell(const GridD &g, const Dev &d, const kz_physics &ph, lo
 // rows 0..ny-1 and writes x ghosts, the y pass reads only domain cells and
 // writes y ghosts, so the two are independent (bc.zig:40-73 runs them in
 // sequence).
-__global__ void __launch_bounds__(64) k_bc_xy(GridD g, Dev d, kz_physics ph) {
-  long t = TID;
-  long nxg = (long)g.nz * g.ny * g.ngx * 2;
-  long nyg = (long)g.nz * g.nx * g.ngy * 2;
-  if (t < nxg) bc_x_cell(g, d, ph, t);
-  else if (t < nxg + nyg) bc_y_cell(g, d, ph, t - nxg);
-}
-
 // bc.zig:286-377 fillCorners2d.  Per corner quadrant the reference copies
 // 2(ng-1) cells from x/y ghosts, averages the diagonal cell next to the
 // domain from an x and a y ghost, then averages the outer diagonal cell from
@@ -299,7 +341,8 @@ __device__ void avg_p2u(const GridD &g, const Dev &d, double gam, long dx_, long
   st(d.p, g.ncell, c, pp);
   st(d.u, g.ncell, c, uu);
 }
-__global__ void __launch_bounds__(64) k_bc_corners2d(GridD g, Dev d, kz_physics ph) {
+// fillCorners2d's two phases, run by one block of at least 20 threads
+__device__ void bc_corners2d(const GridD &g, const Dev &d, const kz_physics &ph) {
   const long nx = g.nx, ny = g.ny, ng = g.ng;
   const double gam = ph.gam;
   const int per = 2 * (g.ng - 1) + 1;  // copies + inner diagonal per quadrant
@@ -324,6 +367,7 @@ __global__ void __launch_bounds__(64) k_bc_corners2d(GridD g, Dev d, kz_physics 
       avg_p2u(g, d, gam, xg1, yg1, xg1, yin, xin, yg1);
     }
   }
+  __threadfence_block();
   __syncthreads();
   if (tid < 4) {
     const bool hx = tid & 2, hy = tid & 1;
@@ -333,55 +377,29 @@ __global__ void __launch_bounds__(64) k_bc_corners2d(GridD g, Dev d, kz_physics 
   }
 }
 
-// ---------------------------------------------------------------------------
-// u2p pass (sim.zig:940-1000)
-__global__ void __launch_bounds__(128) k_u2p(GridD g, Dev d, kz_physics ph) {
+// setBc (bc.zig:40-73): x ghosts (reading only domain cells of rows
+// 0..ny-1) and y ghosts (reading only domain cells) are independent, so one
+// launch does both; the corner fill, which reads both, follows as
+// k_bc_corners.
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_BC) k_bc(GridD g, Dev d, kz_physics ph) {
   long t = TID;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  long c = cidx(g, ix, iy, iz);
-  long n = g.ncell;
-  double uu[NV], pp[NV];
-  ld(d.u, n, c, uu);
-  ld(d.p, n, c, pp);
-  d.flags[KZ_FLAG_ENTROPY * n + c] = 0;
-  d.flags[KZ_FLAG_ENTROPY2 * n + c] = 0;
-  Geo gm;
-  geo_c(g, d, ix, iy, iz, gm);
-  if (polar_row(ph, g, iy)) {
-    double gi = 1.0 / gm.gdet;
-    d.p[B1 * n + c] = uu[B1] * gi;
-    d.p[B2 * n + c] = uu[B2] * gi;
-    d.p[B3 * n + c] = uu[B3] * gi;
-    d.flags[KZ_FLAG_HD_FIXUP * n + c] = 0;
-    d.flags[KZ_FLAG_RAD_FIXUP * n + c] = 0;
-    return;
-  }
-  int ent;
-  int ret = u2p_mhd(uu, pp, gm, ph.gam, ph.floors, &ent);
-  if (ent) d.flags[KZ_FLAG_ENTROPY * n + c] = 1;
-  int corr;
-  u2p_rad(uu, pp, gm, ph.rad, &corr);
-  int guard = 0;
-  check_floors_mhd(pp, gm, ph.gam, ph.floors, &guard);
-  check_floors_rad(pp, gm, ph.rad);
-  st(d.p, n, c, pp);
-  d.flags[KZ_FLAG_HD_FIXUP * n + c] = ret < -1 ? 1 : 0;
-  d.flags[KZ_FLAG_RAD_FIXUP * n + c] = corr ? -1 : 0;
-  if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
+  long nxg = (long)g.nz * g.ny * g.ngx * 2;
+  long nyg = (long)g.nz * g.nx * g.ngy * 2;
+  if (t < nxg) bc_x_cell(g, d, ph, t);
+  else if (t < nxg + nyg) bc_y_cell(g, d, ph, t - nxg);
 }
+__global__ void __launch_bounds__(64) k_bc_corners(GridD g, Dev d, kz_physics ph) { bc_corners2d(g, d, ph); }
 
-// cell_fixup (sim.zig:1027-1133), in place: a flagged cell reads only
-// unflagged neighbours, which no thread writes, so the reference's
-// copy-to-backup staging is unnecessary.
-__global__ void __launch_bounds__(256) k_fixup(GridD g, Dev d, kz_physics ph, int which) {
-  long t = TID;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+// ---------------------------------------------------------------------------
+// cellFixup (sim.zig:1027-1133) for one flagged cell: average the fixed slots
+// from the unflagged in-domain neighbours in the order x-, x+, y-, y+, z-, z+
+// (at least 1/2/3 of them in 1/2/3-D), then p2u.  `which` is one flag; the
+// u2p path applies the hydro then the radiation fixup to a cell in turn,
+// which equals the reference's two passes: they change disjoint slots, and
+// each reads only the slots it changes from cells it does not change.
+__device__ bool fixup_slots(const GridD &g, const Dev &d, const kz_physics &ph, long ix, long iy, long iz, int which,
+                            double *pp) {
   long n = g.ncell;
-  long c = cidx(g, ix, iy, iz);
-  if (d.flags[which * n + c] == 0) return;
-  if (polar_row(ph, g, iy)) return;
   long nx = g.nx, ny = g.ny, nz = g.nz;
   long nb[6][3] = {{ix - 1, iy, iz}, {ix + 1, iy, iz}, {ix, iy - 1, iz}, {ix, iy + 1, iz}, {ix, iy, iz - 1},
                    {ix, iy, iz + 1}};
@@ -394,20 +412,105 @@ __global__ void __launch_bounds__(256) k_fixup(GridD g, Dev d, kz_physics ph, in
     ci[in_n++] = cc;
   }
   bool enough = (nz == 1 && ny == 1 && in_n >= 1) || (nz == 1 && in_n >= 2) || (ny == 1 && in_n >= 2) || in_n >= 3;
-  if (!enough) return;
-  double pp[NV], uu[NV];
-  ld(d.p, n, c, pp);
+  if (!enough) return false;
   for (int iv = 0; iv < NV; ++iv) {
     bool fix;
     if (which == KZ_FLAG_HD_FIXUP) fix = iv != RHO && iv < B1;
     else if (which == KZ_FLAG_RAD_FIXUP) fix = iv >= EE && iv <= FZ;
     else fix = iv != RHO && (iv < B1 || (iv >= EE && iv <= FZ));
     if (fix) {
-      double s = 0;
-      for (int k = 0; k < in_n; ++k) s += d.p[iv * n + ci[k]];
-      pp[iv] = s / (double)in_n;
+      double sum = 0;
+      for (int k = 0; k < in_n; ++k) sum += d.p[iv * n + ci[k]];
+      pp[iv] = sum / (double)in_n;
     }
   }
+  return true;
+}
+__device__ void fixup_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t, bool hd, bool rad) {
+  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  if (polar_row(ph, g, iy)) return;
+  long n = g.ncell;
+  long c = cidx(g, ix, iy, iz);
+  double pp[NV], uu[NV];
+  ld(d.p, n, c, pp);
+  bool changed = false;
+  if (hd && d.flags[KZ_FLAG_HD_FIXUP * n + c] != 0) changed |= fixup_slots(g, d, ph, ix, iy, iz, KZ_FLAG_HD_FIXUP, pp);
+  if (rad && d.flags[KZ_FLAG_RAD_FIXUP * n + c] != 0)
+    changed |= fixup_slots(g, d, ph, ix, iy, iz, KZ_FLAG_RAD_FIXUP, pp);
+  if (!changed) return;
+  Geo gm;
+  geo_c(g, d, ix, iy, iz, gm);
+  p2u(pp, uu, gm, ph.gam);
+  st(d.p, n, c, pp);
+  st(d.u, n, c, uu);
+}
+
+// u2p pass (sim.zig:940-1000); cells flagged for the u2p fixups
+// (sim.zig:1001-1003) are listed for k_u2p_fixups.
+__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_U2P) k_u2p(GridD g, Dev d, kz_physics ph) {
+  long t = TID;
+  long nd = (long)g.nx * g.ny * g.nz;
+  const bool fix_hd = ph.do_fixups, fix_rad = ph.do_fixups && ph.do_u2prad_fixups;
+  if (t < nd) {
+    long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+    long c = cidx(g, ix, iy, iz);
+    long n = g.ncell;
+    double uu[NV], pp[NV];
+    ld(d.u, n, c, uu);
+    ld(d.p, n, c, pp);
+    d.flags[KZ_FLAG_ENTROPY * n + c] = 0;
+    d.flags[KZ_FLAG_ENTROPY2 * n + c] = 0;
+    Geo gm;
+    geo_c(g, d, ix, iy, iz, gm);
+    if (polar_row(ph, g, iy)) {
+      double gi = 1.0 / gm.gdet;
+      d.p[B1 * n + c] = uu[B1] * gi;
+      d.p[B2 * n + c] = uu[B2] * gi;
+      d.p[B3 * n + c] = uu[B3] * gi;
+      d.flags[KZ_FLAG_HD_FIXUP * n + c] = 0;
+      d.flags[KZ_FLAG_RAD_FIXUP * n + c] = 0;
+    } else {
+      int ent;
+      int ret = u2p_mhd(uu, pp, gm, ph.gam, ph.floors, &ent);
+      if (ent) d.flags[KZ_FLAG_ENTROPY * n + c] = 1;
+      int corr;
+      u2p_rad(uu, pp, gm, ph.rad, &corr);
+      int guard = 0;
+      check_floors_mhd(pp, gm, ph.gam, ph.floors, &guard);
+      check_floors_rad(pp, gm, ph.rad);
+      st(d.p, n, c, pp);
+      int fh = ret < -1 ? 1 : 0, fr = corr ? -1 : 0;
+      d.flags[KZ_FLAG_HD_FIXUP * n + c] = fh;
+      d.flags[KZ_FLAG_RAD_FIXUP * n + c] = fr;
+      if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
+      if ((fix_hd && fh) || (fix_rad && fr)) {
+        unsigned k = atomicAdd(&d.sync[SY_NFIX], 1u);
+        d.fix_list[k] = (int32_t)t;
+      }
+    }
+  }
+}
+// the u2p fixups of the listed cells; one block, which then clears the list
+__global__ void __launch_bounds__(256) k_u2p_fixups(GridD g, Dev d, kz_physics ph) {
+  const bool fix_hd = ph.do_fixups, fix_rad = ph.do_fixups && ph.do_u2prad_fixups;
+  unsigned nfix = d.sync[SY_NFIX];
+  for (unsigned i = threadIdx.x; i < nfix; i += blockDim.x) fixup_cell(g, d, ph, d.fix_list[i], fix_hd, fix_rad);
+  __syncthreads();
+  if (threadIdx.x == 0) d.sync[SY_NFIX] = 0;
+}
+
+// the implicit-failure fixup pass (sim.zig:1771-1774), one thread per cell
+__global__ void __launch_bounds__(256) k_fixup(GridD g, Dev d, kz_physics ph, int which) {
+  long t = TID;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  long n = g.ncell;
+  long c = cidx(g, ix, iy, iz);
+  if (d.flags[which * n + c] == 0) return;
+  if (polar_row(ph, g, iy)) return;
+  double pp[NV], uu[NV];
+  ld(d.p, n, c, pp);
+  if (!fixup_slots(g, d, ph, ix, iy, iz, which, pp)) return;
   Geo gm;
   geo_c(g, d, ix, iy, iz, gm);
   p2u(pp, uu, gm, ph.gam);
@@ -418,7 +521,7 @@ __global__ void __launch_bounds__(256) k_fixup(GridD g, Dev d, kz_physics ph, in
 // ---------------------------------------------------------------------------
 // wavespeeds + CFL denominators (sim.zig:747-886)
 __device__ __forceinline__ long long dbits(double x) { return __double_as_longlong(x); }
-__global__ void __launch_bounds__(128) k_wavespeeds(GridD g, Dev d, kz_physics ph) {
+__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_WS) k_wavespeeds(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   int act[3] = {g.nx > 1, g.ny > 1, g.nz > 1};
   long lx = act[0], ly = act[1], lz = act[2];
@@ -492,15 +595,47 @@ __global__ void __launch_bounds__(128) k_wavespeeds(GridD g, Dev d, kz_physics p
 // fused face flux: PPM of the two adjacent cells, face floors, fFluxPrime,
 // p2u, LAXF (sim.zig:1284-1503).  One thread per face i in [0, n] of
 // dimension `dim`, cross indices over [-1, n_c + 1) in active dimensions.
-__global__ void __launch_bounds__(256) k_face_flux(GridD g, Dev d, kz_physics ph, int dim) {
-  long t = TID;
+__device__ __forceinline__ long face_count(const GridD &g, int dim) {
   long nd = dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz);
+  if (nd <= 1) return 0;
   int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
+  long n0 = c0d == 1 ? g.ny : g.nx, n1 = c1d == 1 ? g.ny : g.nz;
+  long w0 = n0 > 1 ? n0 + 2 : n0, w1 = n1 > 1 ? n1 + 2 : n1;
+  return (nd + 1) * w0 * w1;
+}
+// rijvisc.zig:313-353 faceAvg + radvisc.zig:201-244 addRadViscFlux: row
+// dim+1 of the face-averaged viscous R^i_j (rv), velocity-capped, added to
+// the radiation rows of one face state's flux; uu is that state's p2u.  Of
+// the reference's three capped velocities only the face-normal one is used.
+template <int DIM>
+__device__ __forceinline__ void add_visc_flux(const double rv[4], const double *uu, const Geo &gm, const GridD &g,
+                                              const kz_physics &ph, double *ff) {
+  const bool y_active = g.ny > 1, z_active = g.nz > 1;
+  const double gdetu = gm.gdet;
+  double vel = 0;
+#pragma unroll
+  for (int ii = 1; ii < 4; ++ii) {
+    if (ii == 2 && !y_active) continue;
+    if (ii == 3 && !z_active) continue;
+    if (kzm_fabs(uu[EE + ii]) < 1.0e-10 * kzm_fabs(uu[EE])) continue;
+    vel = rv[ii] / (uu[EE + ii] / gdetu) * kzm_sqrt(gm.g[DIM + 1][DIM + 1]);
+  }
+  const double maxv = kzm_fabs(vel);
+  double dampfac = 1.0;
+  if (maxv > ph.radvisc.maxvel) dampfac = ph.radvisc.maxvel / maxv;
+#pragma unroll
+  for (int nu = 0; nu < 4; ++nu) ff[EE + nu] += gdetu * dampfac * rv[nu];
+}
+template <int DIM, bool VISC>
+__device__ __forceinline__ void face_flux(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
+  constexpr int dim = DIM;
+  constexpr int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
+  long nd = dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz);
   long n0 = c0d == 1 ? g.ny : g.nx, n1 = c1d == 1 ? g.ny : g.nz;
   long l0 = n0 > 1 ? -1 : 0, h0 = n0 > 1 ? n0 + 1 : n0;
   long l1 = n1 > 1 ? -1 : 0, h1 = n1 > 1 ? n1 + 1 : n1;
   long w0 = h0 - l0, w1 = h1 - l1, wf = nd + 1;
-  if (t >= wf * w0 * w1) return;
+  if (nd <= 1 || t >= wf * w0 * w1) return;  // a collapsed dimension has no faces
   // x-innermost thread order (coalesced plane loads)
   long i, c0, c1;
   if (dim == 0) {
@@ -537,16 +672,25 @@ __global__ void __launch_bounds__(256) k_face_flux(GridD g, Dev d, kz_physics ph
   }
   Geo gm;
   geo_f(g, d, dim, cell[0], cell[1], cell[2], gm);
+  // each face state: floors, flux, conserved (left state first, then right)
   int guard = 0;
-  check_floors_mhd(pl, gm, ph.gam, ph.floors, &guard);
-  check_floors_mhd(pr, gm, ph.gam, ph.floors, &guard);
   double fll[NV], flr[NV], ul[NV], ur[NV];
-  int e1 = flux_prime(pl, dim, gm, ph.gam, fll);
-  int e2 = flux_prime(pr, dim, gm, ph.gam, flr);
-  if (e1 | e2) set_err(d.err, E_NANFLUX);
+  check_floors_mhd(pl, gm, ph.gam, ph.floors, &guard);
+  int e1 = flux_prime_d<DIM>(pl, gm, ph.gam, fll);
   p2u(pl, ul, gm, ph.gam);
+  check_floors_mhd(pr, gm, ph.gam, ph.floors, &guard);
+  int e2 = flux_prime_d<DIM>(pr, gm, ph.gam, flr);
   p2u(pr, ur, gm, ph.gam);
+  if (e1 | e2) set_err(d.err, E_NANFLUX);
   long cm = cf - stride;
+  if (VISC) {
+    double rv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      rv[j] = 0.5 * (d.rijvisc[((DIM + 1) * 4 + j) * n + cf] + d.rijvisc[((DIM + 1) * 4 + j) * n + cm]);
+    add_visc_flux<DIM>(rv, ul, gm, g, ph, fll);
+    add_visc_flux<DIM>(rv, ur, gm, g, ph, flr);
+  }
   double ag = kzm_fmax(d.scal[(SC_AHD + dim) * n + cf], d.scal[(SC_AHD + dim) * n + cm]);
   double agr = kzm_fmax(d.scal[(SC_ARAD + dim) * n + cf], d.scal[(SC_ARAD + dim) * n + cm]);
   long fi = fidx(g, dim, cell[0], cell[1], cell[2]);
@@ -556,6 +700,21 @@ __global__ void __launch_bounds__(256) k_face_flux(GridD g, Dev d, kz_physics ph
   for (int iv = 0; iv < NV; ++iv) fo[iv * nfd + fi] = laxf(fll[iv], flr[iv], ul[iv], ur[iv], iv >= EE ? agr : ag);
   if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
 }
+// One launch for the faces of every active dimension (faces of different
+// dimensions are independent): threads [0, nf0) do x faces, then y, then z.
+// VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
+template <bool VISC>
+__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, Dev d, kz_physics ph) {
+  long t = TID;
+  long n0 = face_count(g, 0), n1 = face_count(g, 1), n2 = face_count(g, 2);
+  if (t >= n0 + n1 + n2) return;  // the launch's tail threads
+  int dim = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
+  t -= dim == 0 ? 0 : (dim == 1 ? n0 : n0 + n1);
+  if (dim == 0) face_flux<0, VISC>(g, d, ph, t);
+  else if (dim == 1) face_flux<1, VISC>(g, d, ph, t);
+  else face_flux<2, VISC>(g, d, ph, t);
+}
+
 
 // ---------------------------------------------------------------------------
 // flux-CT (ct.zig:38-177)
@@ -599,37 +758,20 @@ __global__ void k_emf(GridD g, Dev d) {
     d.emf[2][e] = 0;
   }
 }
-__global__ void k_ct_rebuild(GridD g, Dev d) {
-  long t = TID;
-  long nx = g.nx, ny = g.ny, nz = g.nz;
-  if (t >= (nx + 1) * (ny + 1) * (nz + 1)) return;
-  long ix = t % (nx + 1), iy = (t / (nx + 1)) % (ny + 1), iz = t / ((nx + 1) * (ny + 1));
-#define EMF(c, a, b, cc) d.emf[(c)-1][eidx(g, a, b, cc)]
-  if (nx > 1 && iy < ny && iz < nz) {
-    long f = fidx(g, 0, ix, iy, iz), m = g.nface[0];
-    d.flb[0][B1 * m + f] = 0;
-    d.flb[0][B2 * m + f] = 0.5 * (EMF(3, ix, iy, iz) + EMF(3, ix, iy + 1, iz));
-    d.flb[0][B3 * m + f] = -0.5 * (EMF(2, ix, iy, iz) + EMF(2, ix, iy, iz + 1));
-  }
-  if (ny > 1 && ix < nx && iz < nz) {
-    long f = fidx(g, 1, ix, iy, iz), m = g.nface[1];
-    d.flb[1][B1 * m + f] = -0.5 * (EMF(3, ix, iy, iz) + EMF(3, ix + 1, iy, iz));
-    d.flb[1][B2 * m + f] = 0;
-    d.flb[1][B3 * m + f] = 0.5 * (EMF(1, ix, iy, iz) + EMF(1, ix, iy, iz + 1));
-  }
-  if (nz > 1 && ix < nx && iy < ny) {
-    long f = fidx(g, 2, ix, iy, iz), m = g.nface[2];
-    d.flb[2][B1 * m + f] = 0.5 * (EMF(2, ix, iy, iz) + EMF(2, ix + 1, iy, iz));
-    d.flb[2][B2 * m + f] = -0.5 * (EMF(1, ix, iy, iz) + EMF(1, ix, iy + 1, iz));
-    d.flb[2][B3 * m + f] = 0;
-  }
-#undef EMF
-}
 
 // ---------------------------------------------------------------------------
 // conserved update: flux divergence + metric source (sim.zig:1140-1175,
-// 1542-1582). dt is read from the step scalars (slot `sslot`).
-__global__ void __launch_bounds__(128) k_update(GridD g, Dev d, kz_physics ph, int sslot) {
+// 1542-1582); dt from the step scalars (slot `sslot`).
+//
+// The B rows of the face fluxes are the flux-CT rebuild of ct.zig:143-177,
+// evaluated here from the corner EMFs with the same expressions instead of
+// being written back to the face arrays first; the hydro/radiation rows come
+// from flb.  The metric source visits (k, l, nu) in the reference order with
+// T^k_l and R^k_l built one row k at a time.
+__device__ __forceinline__ double emf_at(const GridD &g, const Dev &d, int c, long ix, long iy, long iz) {
+  return d.emf[c][eidx(g, ix, iy, iz)];
+}
+__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, Dev d, kz_physics ph, int sslot) {
   long t = TID;
   if (t >= (long)g.nx * g.ny * g.nz) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
@@ -640,56 +782,105 @@ __global__ void __launch_bounds__(128) k_update(GridD g, Dev d, kz_physics ph, i
   geo_c(g, d, ix, iy, iz, gm);
   double pp[NV];
   ld(d.p, n, c, pp);
-  double tij[4][4], tt[4][4], rup[4][4], rij[4][4];
+  double ucon[4], ucov[4], bcon[4], bcov[4], bsq, urf[4];
+  ucon_ucov(pp, gm, ucon, ucov);
+  bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
   {
-    double ucon[4], ucov[4];
-    ucon_ucov(pp, gm, ucon, ucov);
-    double bcon[4], bcov[4], bsq;
-    bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
-    double rho = pp[RHO], u = pp[UU];
-    double pr = (ph.gam - 1.0) * u;
-    double w = rho + u + pr;
-    double eta = w + bsq;
-    double ptot = pr + 0.5 * bsq;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) tij[a][b] = eta * ucon[a] * ucon[b] + ptot * gm.G[a][b] - bcon[a] * bcon[b];
+    double v[4] = {0, pp[FX], pp[FY], pp[FZ]};
+    velr_to_vel4(v, gm, urf);
   }
-  lower_second(tij, gm, tt);
-  calc_rij(pp, gm, rup);
-  lower_second(rup, gm, rij);
+  const double rho = pp[RHO], u = pp[UU], erf = pp[EE];
+  const double pr = (ph.gam - 1.0) * u;
+  const double eta = (rho + u + pr) + bsq;
+  const double ptot = pr + 0.5 * bsq;
   double ms[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) ms[i] = 0;
   const double *kr = d.kris;
-  double gdetu = gm.gdet;
+  const double gdetu = gm.gdet;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 4; ++k) {
+    double trow[4], rrow[4], tk[4], rk[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      trow[m] = eta * ucon[k] * ucon[m] + ptot * gm.G[k][m] - bcon[k] * bcon[m];  // hydro.zig:87-112
+      rrow[m] = four_third * erf * urf[k] * urf[m] + one_third * erf * gm.G[k][m];  // radiation.zig:79-98
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      double st = 0, sr = 0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        st += trow[m] * gm.g[m][l];
+        sr += rrow[m] * gm.g[m][l];
+      }
+      tk[l] = st;
+      rk[l] = sr;
+    }
 #pragma unroll
     for (int l = 0; l < 4; ++l)
 #pragma unroll
       for (int nu = 0; nu < 4; ++nu) {
-        double K = kr[(l * 16 + nu * 4 + k) * n + c];
-        ms[UU + nu] += gdetu * tt[k][l] * K;
-        ms[EE + nu] += gdetu * rij[k][l] * K;
+        const int a = nu < k ? nu : k, b = nu < k ? k : nu;
+        double K = kr[(l * 10 + tri(a, b)) * n + c];
+        ms[UU + nu] += gdetu * tk[l] * K;
+        ms[EE + nu] += gdetu * rk[l] * K;
       }
+  }
   double dx = csize(g, ix, 0), dy = csize(g, iy, 1), dz = csize(g, iz, 2);
   long fx0 = fidx(g, 0, ix, iy, iz), fx1 = fidx(g, 0, ix + 1, iy, iz);
   long fy0 = fidx(g, 1, ix, iy, iz), fy1 = fidx(g, 1, ix, iy + 1, iz);
   long fz0 = fidx(g, 2, ix, iy, iz), fz1 = fidx(g, 2, ix, iy, iz + 1);
   long m0 = g.nface[0], m1 = g.nface[1], m2 = g.nface[2];
+  const bool xa = g.nx > 1, ya = g.ny > 1, za = g.nz > 1;
+  // CT-rebuilt B fluxes (ct.zig:160-175) at the six faces of the cell
+  double bx0[3], bx1[3], by0[3], by1[3], bz0[3], bz1[3];
+  for (int q = 0; q < 2; ++q) {
+    long jx = ix + q, jy = iy + q, jz = iz + q;
+    double *fxb = q ? bx1 : bx0, *fyb = q ? by1 : by0, *fzb = q ? bz1 : bz0;
+    if (xa) {
+      fxb[0] = 0;
+      fxb[1] = 0.5 * (emf_at(g, d, 2, jx, iy, iz) + emf_at(g, d, 2, jx, iy + 1, iz));
+      fxb[2] = -0.5 * (emf_at(g, d, 1, jx, iy, iz) + emf_at(g, d, 1, jx, iy, iz + 1));
+    } else {
+      long f = q ? fx1 : fx0;
+      for (int k = 0; k < 3; ++k) fxb[k] = d.flb[0][(B1 + k) * m0 + f];
+    }
+    if (ya) {
+      fyb[0] = -0.5 * (emf_at(g, d, 2, ix, jy, iz) + emf_at(g, d, 2, ix + 1, jy, iz));
+      fyb[1] = 0;
+      fyb[2] = 0.5 * (emf_at(g, d, 0, ix, jy, iz) + emf_at(g, d, 0, ix, jy, iz + 1));
+    } else {
+      long f = q ? fy1 : fy0;
+      for (int k = 0; k < 3; ++k) fyb[k] = d.flb[1][(B1 + k) * m1 + f];
+    }
+    if (za) {
+      fzb[0] = 0.5 * (emf_at(g, d, 1, ix, iy, jz) + emf_at(g, d, 1, ix + 1, iy, jz));
+      fzb[1] = -0.5 * (emf_at(g, d, 0, ix, iy, jz) + emf_at(g, d, 0, ix, iy + 1, jz));
+      fzb[2] = 0;
+    } else {
+      long f = q ? fz1 : fz0;
+      for (int k = 0; k < 3; ++k) fzb[k] = d.flb[2][(B1 + k) * m2 + f];
+    }
+  }
 #pragma unroll
   for (int iv = 0; iv < NV; ++iv) {
-    double du = -(d.flb[0][iv * m0 + fx1] - d.flb[0][iv * m0 + fx0]) * dt / dx -
-                (d.flb[1][iv * m1 + fy1] - d.flb[1][iv * m1 + fy0]) * dt / dy -
-                (d.flb[2][iv * m2 + fz1] - d.flb[2][iv * m2 + fz0]) * dt / dz;
+    double fxl, fxr, fyl, fyr, fzl, fzr;
+    if (iv >= B1 && iv <= B3) {
+      fxl = bx0[iv - B1]; fxr = bx1[iv - B1];
+      fyl = by0[iv - B1]; fyr = by1[iv - B1];
+      fzl = bz0[iv - B1]; fzr = bz1[iv - B1];
+    } else {
+      fxl = d.flb[0][iv * m0 + fx0]; fxr = d.flb[0][iv * m0 + fx1];
+      fyl = d.flb[1][iv * m1 + fy0]; fyr = d.flb[1][iv * m1 + fy1];
+      fzl = d.flb[2][iv * m2 + fz0]; fzr = d.flb[2][iv * m2 + fz1];
+    }
+    double du = -(fxr - fxl) * dt / dx - (fyr - fyl) * dt / dy - (fzr - fzl) * dt / dz;
     double uo = d.u[iv * n + c];
     d.u[iv * n + c] = uo + du + ms[iv] * dt;
   }
 }
 
-// ---------------------------------------------------------------------------
 // implicit four-force solve (sim.zig:1736-1776 -> implicit.zig:927-1004),
 // split in three kernels so that each keeps its state in registers:
 //   k_imp_prep     reference state, bisection warm start, starting block
@@ -703,7 +894,7 @@ __global__ void __launch_bounds__(128) k_update(GridD g, Dev d, kz_physics ph, i
 __device__ __forceinline__ long dom_t(const GridD &g, long ix, long iy, long iz) {
   return ix + (long)g.nx * (iy + (long)g.ny * iz);
 }
-__global__ void __launch_bounds__(64) k_imp_prep(GridD g, Dev d, kz_physics ph, int sslot) {
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(GridD g, Dev d, kz_physics ph, int sslot) {
   long t = TID;
   if (t == 0) *d.imp_nslow = 0;  // no appends run concurrently with this kernel
   long nd = (long)g.nx * g.ny * g.nz;
@@ -766,7 +957,7 @@ __device__ __forceinline__ void imp_commit(const GridD &g, const Dev &d, const k
   atomicAdd(&d.cnt[2], 1ULL);
 }
 template <int WP>
-__global__ void __launch_bounds__(64) k_imp_rung0(GridD g, Dev d, kz_physics ph, int sslot) {
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(GridD g, Dev d, kz_physics ph, int sslot) {
   long t = TID;
   long nd = (long)g.nx * g.ny * g.nz;
   if (t >= nd) return;
@@ -859,39 +1050,476 @@ __global__ void k_polar(GridD g, Dev d, kz_physics ph) {
 }
 
 // ---------------------------------------------------------------------------
-// stage arithmetic over domain cells (exact reference shapes)
-__global__ void k_stage_deriv(GridD g, double *dst, const double *a, const double *b, const double *ss, int f1s,
-                              int f2s) {
-  long t = TID;
-  long nd = (long)g.nx * g.ny * g.nz;
-  if (t >= nd * NV) return;
-  long cellt = t % nd;
-  int iv = t / nd;
-  long ix = cellt % g.nx, iy = (cellt / g.nx) % g.ny, iz = cellt / ((long)g.nx * g.ny);
-  long o = iv * g.ncell + cidx(g, ix, iy, iz);
-  dst[o] = ss[f1s] * a[o] + ss[f2s] * b[o];
+// radiative shear viscosity (sim/rijvisc.zig, physics/radvisc.zig)
+__device__ __forceinline__ double krv(const Dev &d, long n, long c, int l, int a, int b) {
+  return d.kris[(l * 10 + (a < b ? tri(a, b) : tri(b, a))) * n + c];
 }
-__global__ void k_stage_combine(GridD g, double *dst, const double *a, const double *b, const double *c,
-                                const double *ss, int f1s, int f2s) {
-  long t = TID;
-  long nd = (long)g.nx * g.ny * g.nz;
-  if (t >= nd * NV) return;
-  long cellt = t % nd;
-  int iv = t / nd;
-  long ix = cellt % g.nx, iy = (cellt / g.nx) % g.ny, iz = cellt / ((long)g.nx * g.ny);
-  long o = iv * g.ncell + cidx(g, ix, iy, iz);
-  dst[o] = a[o] + ss[f1s] * b[o] + ss[f2s] * c[o];
+// math/misc.zig:9-12 stepFunction
+__device__ __forceinline__ double step_function(double x, double x9) {
+  double k = 1.47222 / x9;
+  return 1.0 / (1.0 + kzm_exp(-2.0 * k * x));
 }
-__global__ void k_copy_entropy_flag(GridD g, Dev d) {
+// relele.convertBoth(VELR) of the radiation-frame velocity of cell c
+__device__ __forceinline__ void rad_vel_both(const Dev &d, long n, long c, const Geo &gm, double con[4],
+                                             double cov[4]) {
+  double v[4] = {0.0, d.p[FX * n + c], d.p[FY * n + c], d.p[FZ * n + c]};
+  velr_to_vel4(v, gm, con);
+  lower4(con, gm, cov);
+}
+__device__ __forceinline__ bool is_corner(const GridD &g, long ix, long iy, long iz) {  // sim.zig:660-666
+  int k = 0;
+  if (ix < 0 || ix >= g.nx) k += 1;
+  if (iy < 0 || iy >= g.ny) k += 1;
+  if (iz < 0 || iz >= g.nz) k += 1;
+  return k >= 2;
+}
+__device__ __forceinline__ double ccoord(const GridD &g, int idim, long i) {
+  return idim == 1 ? xc_(g, i) : (idim == 2 ? yc_(g, i) : zc_(g, i));
+}
+// rijvisc.zig:46-172 calcShearLab then radvisc.zig:72-131
+// shearFromGradients: sigma_{mu nu} of the radiation frame in the lab frame,
+// centred differences except the one-sided ones next to the skipped corners.
+__device__ void shear_lab(const GridD &g, const Dev &d, long ix, long iy, long iz, const Geo &geom,
+                          double sig[4][4]) {
+  const long n = g.ncell, c = cidx(g, ix, iy, iz);
+  double du[4][4], du2[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) du[i][j] = du2[i][j] = 0.0;
+  double ucon[4], ucov[4];
+  rad_vel_both(d, n, c, geom, ucon, ucov);
+  const long nd[3] = {g.nx, g.ny, g.nz};
+  const long nx = g.nx, ny = g.ny, nz = g.nz;
+#pragma unroll 1
+  for (int idim = 1; idim < 4; ++idim) {
+    const int dim = idim - 1;
+    if (nd[dim] <= 1) continue;
+    long cm[3] = {ix, iy, iz}, cp[3] = {ix, iy, iz};
+    cm[dim] -= 1;
+    cp[dim] += 1;
+    double umc[4], umv[4], upc[4], upv[4];
+    {
+      Geo gx;
+      geo_c(g, d, cm[0], cm[1], cm[2], gx);
+      rad_vel_both(d, n, cidx(g, cm[0], cm[1], cm[2]), gx, umc, umv);
+      geo_c(g, d, cp[0], cp[1], cp[2], gx);
+      rad_vel_both(d, n, cidx(g, cp[0], cp[1], cp[2]), gx, upc, upv);
+    }
+    const double xm = ccoord(g, idim, cm[dim]), xc = ccoord(g, idim, dim == 0 ? ix : (dim == 1 ? iy : iz)),
+                 xp = ccoord(g, idim, cp[dim]);
+    // 0 centred, 1 right-sided, 2 left-sided
+    int side = 0;
+    if ((ix < 0 && iy == 0 && iz == 0 && idim != 1) || (iy < 0 && ix == 0 && iz == 0 && idim != 2) ||
+        (iz < 0 && ix == 0 && iy == 0 && idim != 3))
+      side = 1;
+    else if ((ix < 0 && iy == ny - 1 && iz == nz - 1 && idim != 1) ||
+             (iy < 0 && ix == nx - 1 && iz == nz - 1 && idim != 2) ||
+             (iz < 0 && ix == nx - 1 && iy == ny - 1 && idim != 3))
+      side = 2;
+    else if ((ix >= nx && iy == 0 && iz == 0 && idim != 1) || (iy >= ny && ix == 0 && iz == 0 && idim != 2) ||
+             (iz >= nz && ix == 0 && iy == 0 && idim != 3))
+      side = 1;
+    else if ((ix >= nx && iy == ny - 1 && iz == nz - 1 && idim != 1) ||
+             (iy >= ny && ix == nx - 1 && iz == nz - 1 && idim != 2) ||
+             (iz >= nz && ix == nx - 1 && iy == ny - 1 && idim != 3))
+      side = 2;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double a, b;
+      if (side == 1) {
+        a = (upv[i] - ucov[i]) / (xp - xc);
+        b = (upc[i] - ucon[i]) / (xp - xc);
+      } else if (side == 2) {
+        a = (ucov[i] - umv[i]) / (xc - xm);
+        b = (ucon[i] - umc[i]) / (xc - xm);
+      } else {
+        a = (upv[i] - umv[i]) / (xp - xm);
+        b = (upc[i] - umc[i]) / (xp - xm);
+      }
+      // du[i][idim], du2[i][idim] with a register-friendly column select
+#pragma unroll
+      for (int q = 1; q < 4; ++q)
+        if (q == idim) {
+          du[i][q] = a;
+          du2[i][q] = b;
+        }
+    }
+  }
+  double dcu[4][4], dcudiag[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double krsum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) krsum += krv(d, n, c, k, i, j) * ucov[k];
+      dcu[i][j] = du[i][j] - krsum;
+    }
+    double krsum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) krsum += krv(d, n, c, i, i, k) * ucon[k];
+    dcudiag[i] = du2[i][i] + krsum;
+  }
+  double theta = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) theta += dcudiag[i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sig[i][j] = 0.0;
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+#pragma unroll
+    for (int j = 1; j < 4; ++j) {
+      double sum1 = 0, sum2 = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sum1 += dcu[i][k] * ((k == j ? 1.0 : 0.0) + ucon[k] * ucov[j]);
+        sum2 += dcu[j][k] * ((k == i ? 1.0 : 0.0) + ucon[k] * ucov[i]);
+      }
+      sig[i][j] = 0.5 * (sum1 + sum2) - 1.0 / 3.0 * theta * (geom.g[i][j] + ucov[i] * ucov[j]);
+    }
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    double v = -1.0 / ucon[0] * (ucon[1] * sig[1][i] + ucon[2] * sig[2][i] + ucon[3] * sig[3][i]);
+    sig[i][0] = v;
+    sig[0][i] = v;
+  }
+  sig[0][0] = -1.0 / ucon[0] * (ucon[1] * sig[1][0] + ucon[2] * sig[2][0] + ucon[3] * sig[3][0]);
+}
+// radvisc.zig:154-175 viscCoeff
+__device__ __forceinline__ double visc_coeff(double chi, double mindx, double r_bl, double rhor, double alpha,
+                                             double global_dt) {
+  double mfp = 1.0 / chi;
+  const double rmin = 1.2 * rhor;
+  const double mfplim = r_bl;
+  if (mfp > mfplim || chi < 1.0e-80) mfp = mfplim;
+  if (mfp < 0 || !kzm_isfinite(mfp)) mfp = 0;
+  mfp *= step_function(r_bl - rmin, 0.2 * rmin);
+  if (r_bl <= 1.0 * rhor) mfp = 0;
+  double nu = alpha * mfp;
+  const double nulimit = mindx * mindx / 2.0 / global_dt / 2.0;
+  if (nu > nulimit) nu = nulimit;
+  return nu;
+}
+// rijvisc.zig:266-311 calcRijViscTotal -> calcRijVisc (:175-264): one
+// thread per cell of the domain and its one-cell ring, corners skipped (their
+// R^i_j stays 0).  global_dt is the step-scalar slot `sslot`.
+__global__ void __launch_bounds__(64) k_rijvisc(GridD g, Dev d, kz_physics ph, int sslot, double rhor) {
   long t = TID;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const long lx = g.nx > 1, ly = g.ny > 1, lz = g.nz > 1;
+  const long wx = g.nx + 2 * lx, wy = g.ny + 2 * ly, wz = g.nz + 2 * lz;
+  if (t >= wx * wy * wz) return;
+  const long ix = t % wx - lx, iy = (t / wx) % wy - ly, iz = t / (wx * wy) - lz;
+  if (is_corner(g, ix, iy, iz)) return;
+  const long n = g.ncell, c = cidx(g, ix, iy, iz);
+  const double global_dt = d.ss[sslot];
+  Geo geom;
+  geo_c(g, d, ix, iy, iz, geom);
+  double sl[4][4];
+  shear_lab(g, d, ix, iy, iz, geom, sl);
+  // relele.zig:115-128 raiseBoth
+  double su[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) acc += sl[k][l] * geom.G[i][k] * geom.G[j][l];
+      su[i][j] = acc;
+    }
+  double pp[NV];
+  ld(d.p, n, c, pp);
+  const double chi = calc_chi_slim(pp, geom, ph.gam, ph.opac);
+  const double dx0 = csize(g, ix, 0) * kzm_sqrt(geom.g[1][1]), dx1 = csize(g, iy, 1) * kzm_sqrt(geom.g[2][2]),
+               dx2 = csize(g, iz, 2) * kzm_sqrt(geom.g[3][3]);
+  const double mindx = (g.ny == 1 && g.nz == 1) ? dx0
+                       : (g.nz == 1)            ? kzm_fmin(dx0, dx1)
+                       : (g.ny == 1)            ? kzm_fmin(dx0, dx2)
+                                                : kzm_fmin(dx0, kzm_fmin(dx1, dx2));
+  const double nu = visc_coeff(chi, mindx, d.bl[BL_R * n + c], rhor, ph.radvisc.alpha, global_dt);
+  double rv[4][4], lo[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rv[i][j] = -2.0 * nu * pp[EE] * su[i][j];
+  lower_second(rv, geom, lo);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d.rijvisc[(i * 4 + j) * n + c] = lo[i][j];
+}
+
+// ---------------------------------------------------------------------------
+// mean-field dynamo (sim/dynamo.zig)
+__device__ __forceinline__ void bl_geo(const Dev &d, long n, long c, Geo &o) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      o.g[i][j] = d.bl[(BL_G + i * 4 + j) * n + c];
+      o.G[i][j] = d.bl[(BL_GC + i * 4 + j) * n + c];
+    }
+  o.alpha = d.bl[BL_ALPHA * n + c];
+  o.gdet = d.bl[BL_GDET * n + c];
+  o.gttpert = 0;
+  o.x1 = d.bl[BL_R * n + c];
+}
+// dynamo.zig:226-256 calcScaleHeight (single rank): per radial column the
+// density-weighted RMS of pi/2 - theta, summed in the reference's row order;
+// column 0 keeps the raw sum (the reference's gix > 0 condition).
+__global__ void __launch_bounds__(64) k_scale_height(GridD g, Dev d) {
+  const long ix = TID;
+  if (ix >= g.nx) return;
+  const long n = g.ncell;
+  double sigma = 0, sth = 0;
+  for (long iy = 0; iy < g.ny; ++iy)
+    for (long iz = 0; iz < g.nz; ++iz) {
+      const long c = cidx(g, ix, iy, iz);
+      const double gd = d.gc[26 * n + c];
+      const double th = d.bl[BL_TH * n + c];
+      const double rho = d.p[RHO * n + c];
+      const double dth = KZ_PI / 2.0 - th;
+      sigma += rho * gd;
+      sth += rho * gd * dth * dth;
+    }
+  d.scaleth[ix] = ix > 0 ? kzm_sqrt(sth / sigma) : sth;
+}
+// dynamo.zig:67-73 dampBphi
+__device__ __forceinline__ double damp_bphi(double alphabeta, double facradius, double faczh, double dt_over_pk,
+                                            double beta, double betasat, double bphi) {
+  double dbphi = -alphabeta * facradius * faczh * dt_over_pk * kzm_fmax(0.0, beta - betasat) / betasat * bphi;
+  if ((dbphi + bphi) * bphi < 0.0) dbphi = -bphi;
+  return dbphi;
+}
+// dynamo.zig:104-151 dynamoDeltaA (the caller skips r < 1.0001 r_horizon)
+__device__ double dynamo_delta_a(const kz_dynamo_params &dp, double a, double dt, double risco, double r, double th,
+                                 double bphi, double bsq, double angle, double prermhd, double gg33, double scaleth,
+                                 double *bphi_out) {
+  *bphi_out = bphi;
+  const double omk = 1.0 / (a + kzm_sqrt(r * r * r));
+  const double pk = 2.0 * KZ_PI / omk;
+  double facangle = 0;
+  if (kzm_isfinite(angle)) {
+    if (angle < -1.0) angle = -1.0;
+    facangle = kzm_fmax(0.0, (dp.thetaangle - angle) / dp.thetaangle);
+  }
+  const double facradius = step_function(r - 1.0 * risco, 0.1 * risco);
+  const double beta = 0.5 * bsq / prermhd;
+  double hrdtheta;
+  if (dp.calchronthego) {
+    hrdtheta = scaleth;
+    if (hrdtheta > 0.9 * KZ_PI / 2.0) hrdtheta = 0.9 * KZ_PI / 2.0;
+  } else {
+    hrdtheta = dp.expectedhr * KZ_PI / 2.0;
+  }
+  const double zh = (KZ_PI / 2.0 - th) / hrdtheta;
+  const double faczh = kzm_fmax(0.0, 1.0 - zh * zh);
+  const double facmagnetization = faczh;
+  double effalpha = dp.alphadynamo;
+  if (dp.alphaflipssign) effalpha = -(KZ_PI / 2.0 - th) / (hrdtheta / 2.0) * dp.alphadynamo;
+  const double aphi = effalpha * (hrdtheta / (KZ_PI / 2.0)) / 0.4 * dt / pk * r * gg33 * bphi * facradius *
+                      facmagnetization * facangle;
+  if (dp.dampbeta)
+    *bphi_out = bphi + damp_bphi(dp.alphabeta, facradius, faczh, dt / pk, beta, dp.betasaturated, bphi);
+  return aphi;
+}
+// dynamo.zig:370-430 mimicDynamo, first half: Delta A_phi on the domain and
+// its one-cell ring (corners included) with the field angle of
+// dynamo.zig:335-368 evaluated in BL; DAMPBETA rewrites p[B3] in place.
+__global__ void __launch_bounds__(64) k_dynamo_a(GridD g, Dev d, kz_physics ph, int sslot, double rhor,
+                                                 double risco) {
+  long t = TID;
+  const long lx = g.nx > 1, ly = g.ny > 1, lz = g.nz > 1;
+  const long wx = g.nx + 2 * lx, wy = g.ny + 2 * ly, wz = g.nz + 2 * lz;
+  if (t >= wx * wy * wz) return;
+  const long ix = t % wx - lx, iy = (t / wx) % wy - ly, iz = t / (wx * wy) - lz;
+  const long n = g.ncell, c = cidx(g, ix, iy, iz);
+  const double dt = d.ss[sslot];
+  Geo geom, gbl;
+  geo_c(g, d, ix, iy, iz, geom);
+  bl_geo(d, n, c, gbl);
+  double pp[NV], ppbl[NV];
+  ld(d.p, n, c, pp);
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) ppbl[iv] = pp[iv];
+  double J[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) J[k] = d.bl[(BL_J + k) * n + c];
+  const int e = trans_pmhd(ppbl, geom, gbl, J);
+  if (e) {
+    set_err(d.err, e);
+    return;
+  }
+  double angle, bsq;
+  {
+    double ucon[4], ucov[4], bcon[4], bcov[4], b2;
+    ucon_ucov(ppbl, gbl, ucon, ucov);
+    bcon_bcov_bsq(ppbl[B1], ppbl[B2], ppbl[B3], ucon, ucov, gbl, bcon, bcov, &b2);
+    const double brbphi = kzm_sqrt(gbl.g[1][1] * gbl.g[3][3]) * bcon[1] * bcon[3];
+    angle = -brbphi / b2;
+    bsq = b2;
+  }
+  const double r = d.bl[BL_R * n + c], th = d.bl[BL_TH * n + c];
+  if (r < 1.0001 * rhor) {
+    d.dyn_a[c] = 0.0;
+    return;
+  }
+  double prermhd = (ph.gam - 1.0) * pp[UU];
+  const double ehat = -calc_ff_rtt(pp, geom);
+  prermhd += ehat / 3.0;
+  const long gix = ix < 0 ? 0 : (ix > g.nx - 1 ? g.nx - 1 : ix);
+  double bphi_new;
+  const double aphi = dynamo_delta_a(ph.dynamo_params, ph.bh_spin, dt, risco, r, th, pp[B3], bsq, angle, prermhd,
+                                     geom.g[3][3], d.scaleth[gix], &bphi_new);
+  d.dyn_a[c] = aphi;
+  if (ph.dynamo_params.dampbeta) d.p[B3 * n + c] = bphi_new;
+}
+// mimicDynamo, second half (2D): the corner average of (0, 0, Delta A_phi)
+// (ct.zig:228-264 cornerAverageA), its curl (ct.zig:279-331 calcBfromACore)
+// and the superposition on the domain's B (dynamo.zig:432-460), with u[B]
+// from p2u.  The corner values are formed per cell from the four cells
+// around each corner, the reference's sum order.
+__device__ __forceinline__ double corner_a(const GridD &g, const Dev &d, long ix, long iy) {
+  return 1.0 / 4.0 *
+         (d.dyn_a[cidx(g, ix, iy, 0)] + d.dyn_a[cidx(g, ix, iy - 1, 0)] + d.dyn_a[cidx(g, ix - 1, iy, 0)] +
+          d.dyn_a[cidx(g, ix - 1, iy - 1, 0)]);
+}
+__global__ void __launch_bounds__(128) k_dynamo_curl(GridD g, Dev d, kz_physics ph) {
+  long t = TID;
+  if (t >= (long)g.nx * g.ny) return;
+  const long ix = t % g.nx, iy = t / g.nx;
+  const long n = g.ncell, c = cidx(g, ix, iy, 0);
+  const double z = 1.0 / 4.0 * (0.0 + 0.0 + 0.0 + 0.0);
+  const double acorner[4] = {corner_a(g, d, ix, iy), corner_a(g, d, ix, iy + 1), corner_a(g, d, ix + 1, iy),
+                             corner_a(g, d, ix + 1, iy + 1)};
+  double dA[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dA[i][j] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double a00 = k == 2 ? acorner[0] : z, a01 = k == 2 ? acorner[1] : z, a10 = k == 2 ? acorner[2] : z,
+                 a11 = k == 2 ? acorner[3] : z;
+    dA[k + 1][2] = -(a00 - a01 + a10 - a11) / (2.0 * csize(g, iy, 1));
+    dA[k + 1][1] = -(a00 + a01 - a10 - a11) / (2.0 * csize(g, ix, 0));
+    dA[k + 1][3] = 0;
+  }
+  const double gdet = d.gc[26 * n + c];
+  const double v3 = (dA[2][3] - dA[3][2]) / gdet;
+  const double v4 = (dA[3][1] - dA[1][3]) / gdet;
+  const double v5 = (dA[1][2] - dA[2][1]) / gdet;
+  double pp[NV], uu[NV];
+  ld(d.p, n, c, pp);
+  pp[B1] += v3;
+  pp[B2] += v4;
+  pp[B3] += v5;
+  Geo geom;
+  geo_c(g, d, ix, iy, 0, geom);
+  p2u(pp, uu, geom, ph.gam);
+  d.p[B1 * n + c] = pp[B1];
+  d.p[B2 * n + c] = pp[B2];
+  d.p[B3 * n + c] = pp[B3];
+  d.u[B1 * n + c] = uu[B1];
+  d.u[B2 * n + c] = uu[B2];
+  d.u[B3 * n + c] = uu[B3];
+}
+
+// stage arithmetic over domain cells (sim.zig:1586-1709), fused per step
+// phase.  Each kernel applies, per element, the reference's expressions in
+// the reference's order (stageDeriv f1*a + f2*b, stageCombine a + f1*b +
+// f2*c).  The stage copies ut0/ut1/uforget/ut2 are only ever read on domain
+// cells, so only domain cells are copied.
+__device__ __forceinline__ bool dom_cell(const GridD &g, long t, long &c) {
+  if (t >= (long)g.nx * g.ny * g.nz) return false;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  long c = cidx(g, ix, iy, iz);
-  d.flags[KZ_FLAG_ENTROPY3 * g.ncell + c] = d.flags[KZ_FLAG_ENTROPY * g.ncell + c];
+  c = cidx(g, ix, iy, iz);
+  return true;
 }
-// u2p.c update_entropy (sim.zig:1683-1709)
-__global__ void k_update_entropy(GridD g, Dev d, kz_physics ph) {
+// step start (sim.zig:1792-1830): own_dt = 1/tstepdenmax, dt, the stage
+// coefficients, the CFL accumulators reset; ut0 = u.
+__global__ void k_begin_step(GridD g, Dev d, double forced, int use_forced) {
+  long t = TID, c;
+  if (t == 0) {
+    double *ss = d.ss;
+    double tmax = __longlong_as_double((long long)d.tsd[0]);
+    double own = 1.0 / tmax;
+    double dt = use_forced ? forced : own;
+    const double gm = GAM_IMEX;
+    ss[SS_OWN_DT] = own;
+    ss[SS_DT] = dt;
+    ss[SS_DTG] = dt * gm;
+    ss[SS_INV_DTG] = 1.0 / (dt * gm);
+    ss[SS_NINV_DTG] = -1.0 / (dt * gm);
+    ss[SS_INV_DT] = 1.0 / dt;
+    ss[SS_NINV_DT] = -1.0 / dt;
+    ss[SS_C12] = dt * (1.0 - 2.0 * gm);
+    ss[SS_HALF] = dt / 2.0;
+    d.tsd[0] = (unsigned long long)__double_as_longlong(-1.0);
+    d.tsd[1] = (unsigned long long)__double_as_longlong(1.0e50);
+  }
+  if (!dom_cell(g, t, c)) return;
+  long n = g.ncell;
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) d.ut0[iv * n + c] = d.u[iv * n + c];
+}
+// after an implicit stage: drt = u/(dt g) - base/(dt g); next stage copy = u
+__global__ void k_stage_implicit(GridD g, Dev d, double *drt, const double *base, double *copy) {
+  long t = TID, c;
+  if (!dom_cell(g, t, c)) return;
+  long n = g.ncell;
+  const double f1 = d.ss[SS_INV_DTG], f2 = d.ss[SS_NINV_DTG];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    long o = iv * n + c;
+    double u = d.u[o];
+    drt[o] = f1 * u + f2 * base[o];
+    copy[o] = u;
+  }
+}
+// after the first explicit stage: copyEntropyCount, dut1, the first
+// together-combine u = ut0 + dt dut1 + dt(1-2g) drt1, uforget = u
+__global__ void k_stage_explicit1(GridD g, Dev d) {
+  long t = TID, c;
+  if (!dom_cell(g, t, c)) return;
+  long n = g.ncell;
+  d.flags[KZ_FLAG_ENTROPY3 * n + c] = d.flags[KZ_FLAG_ENTROPY * n + c];
+  const double f1 = d.ss[SS_INV_DT], f2 = d.ss[SS_NINV_DT], dt = d.ss[SS_DT], c12 = d.ss[SS_C12];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    long o = iv * n + c;
+    double du = f1 * d.u[o] + f2 * d.ut1[o];
+    d.dut1[o] = du;
+    double u = d.ut0[o] + dt * du + c12 * d.drt1[o];
+    d.u[o] = u;
+    d.uforget[o] = u;
+  }
+}
+// after the second explicit stage: dut2, u = ut0 + dt/2 dut1 + dt/2 dut2,
+// then u += dt/2 drt1 + dt/2 drt2
+__global__ void k_stage_explicit2(GridD g, Dev d) {
+  long t = TID, c;
+  if (!dom_cell(g, t, c)) return;
+  long n = g.ncell;
+  const double f1 = d.ss[SS_INV_DT], f2 = d.ss[SS_NINV_DT], h = d.ss[SS_HALF];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    long o = iv * n + c;
+    double du2 = f1 * d.u[o] + f2 * d.ut2[o];
+    d.dut2[o] = du2;
+    double u = d.ut0[o] + h * d.dut1[o] + h * du2;
+    d.u[o] = u + h * d.drt1[o] + h * d.drt2[o];
+  }
+}
+// step end: t += dt, then updateEntropy (sim.zig:1683-1709)
+__global__ void k_end_step(GridD g, Dev d, kz_physics ph) {
   long t = TID;
+  if (t == 0) d.ss[SS_T] = d.ss[SS_T] + d.ss[SS_DT];
   if (t >= (long)g.nx * g.ny * g.nz) return;
   long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
   long n = g.ncell;
@@ -907,29 +1535,6 @@ __global__ void k_update_entropy(GridD g, Dev d, kz_physics ph) {
 #pragma unroll
   for (int iv = 0; iv <= B3; ++iv) d.u[iv * n + c] = uu[iv];
 }
-
-// step scalars (sim.zig:1792-1830): own_dt = 1/tstepdenmax, dt = forced or
-// own, and the stage coefficients with the reference's expression shapes;
-// then the CFL accumulators are reset.
-__global__ void k_begin_step(Dev d, double forced, int use_forced) {
-  double *ss = d.ss;
-  double tmax = __longlong_as_double((long long)d.tsd[0]);
-  double own = 1.0 / tmax;
-  double dt = use_forced ? forced : own;
-  const double gm = GAM_IMEX;
-  ss[SS_OWN_DT] = own;
-  ss[SS_DT] = dt;
-  ss[SS_DTG] = dt * gm;
-  ss[SS_INV_DTG] = 1.0 / (dt * gm);
-  ss[SS_NINV_DTG] = -1.0 / (dt * gm);
-  ss[SS_INV_DT] = 1.0 / dt;
-  ss[SS_NINV_DT] = -1.0 / dt;
-  ss[SS_C12] = dt * (1.0 - 2.0 * gm);
-  ss[SS_HALF] = dt / 2.0;
-  d.tsd[0] = (unsigned long long)__double_as_longlong(-1.0);
-  d.tsd[1] = (unsigned long long)__double_as_longlong(1.0e50);
-}
-__global__ void k_end_step(Dev d) { d.ss[SS_T] = d.ss[SS_T] + d.ss[SS_DT]; }
 
 // postinit beta normalisation (puffy.zig:892-1024)
 __global__ void k_beta_max(GridD g, Dev d, kz_physics ph, unsigned long long *out) {
@@ -1015,6 +1620,10 @@ struct kz_sim {
   double t_host;
   uint64_t nstep;
   std::vector<void *> allocs;
+  // BL horizon and ISCO radii (metric.zig:20-32) and whether the BL sidecar
+  // has been loaded (radiative viscosity and dynamo need it)
+  double rhor = 0, risco = 0;
+  bool has_bl = false;
   // event timing of one kernel class (kz_sim_time_kernel)
   int timed = -1;
   std::vector<hipEvent_t> ev_pool;  // created events, reused
@@ -1065,19 +1674,44 @@ static int dalloc(kz_sim *s, void **p, size_t bytes) {
 
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// repack reference 20-blocks into the 35 SoA planes: g[16] G[16] gdet gttpert alpha
-static void pack_geo(const double *g20, const double *G20, long n, std::vector<double> &out) {
+static inline uint64_t bits_of(double x) {
+  uint64_t b;
+  memcpy(&b, &x, sizeof b);
+  return b;
+}
+// Repack the reference's 20-double g/G blocks into the GEO_NC SoA planes.
+// Returns false if a g block is not exactly symmetric.
+static bool pack_geo(const double *g20, const double *G20, long n, std::vector<double> &out) {
   out.assign((size_t)n * GEO_NC, 0.0);
   for (long c = 0; c < n; ++c) {
+    const double *g = g20 + c * 20, *G = G20 + c * 20;
     for (int a = 0; a < 4; ++a)
-      for (int b = 0; b < 4; ++b) {
-        out[(a * 4 + b) * n + c] = g20[c * 20 + a * 5 + b];
-        out[(16 + a * 4 + b) * n + c] = G20[c * 20 + a * 5 + b];
+      for (int b = a; b < 4; ++b) {
+        if (bits_of(g[a * 5 + b]) != bits_of(g[b * 5 + a])) return false;
+        out[(size_t)tri(a, b) * n + c] = g[a * 5 + b];
       }
-    out[32 * n + c] = g20[c * 20 + 19];
-    out[33 * n + c] = G20[c * 20 + 19];
-    out[34 * n + c] = __builtin_sqrt(-1.0 / G20[c * 20 + 0]);  // precompute.zig:494
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) out[(size_t)(10 + a * 4 + b) * n + c] = G[a * 5 + b];
+    out[26 * n + c] = g[19];
+    out[27 * n + c] = G[19];
+    out[28 * n + c] = __builtin_sqrt(-1.0 / G[0]);  // precompute.zig:494
   }
+  return true;
+}
+// Christoffel blocks kr[l*16 + nu*4 + k] -> KR_NC planes (l, tri(nu, k));
+// false if a block is not exactly symmetric in (nu, k).
+static bool pack_kris(const double *kris, long n, std::vector<double> &out) {
+  out.assign((size_t)n * KR_NC, 0.0);
+  for (long c = 0; c < n; ++c) {
+    const double *k = kris + c * 64;
+    for (int l = 0; l < 4; ++l)
+      for (int a = 0; a < 4; ++a)
+        for (int b = a; b < 4; ++b) {
+          if (bits_of(k[l * 16 + a * 4 + b]) != bits_of(k[l * 16 + b * 4 + a])) return false;
+          out[(size_t)(l * 10 + tri(a, b)) * n + c] = k[l * 16 + a * 4 + b];
+        }
+  }
+  return true;
 }
 
 extern "C" {
@@ -1087,9 +1721,10 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
                       const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
   if (gr->nz != 1 || gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return nullptr; }
-  if (ph->radviscosity || ph->dynamo || ph->reduceorderatbh || ph->reduceorderafterfixup ||
-      ph->opac.synchrotron_bridge || ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return nullptr; }
+  if (ph->reduceorderatbh || ph->reduceorderafterfixup || ph->opac.synchrotron_bridge ||
+      ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return nullptr; }
   if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
+  if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
   kz_sim *s = new kz_sim();
   s->ph = *ph;
   GridD &G = s->g;
@@ -1108,6 +1743,17 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
     G.fsy[d] = G.sy + (d == 1);
     G.nface[d] = (long)G.fsx[d] * G.fsy[d] * (G.sz + (d == 2));
   }
+  // repack and check the metric tables before touching the device
+  std::vector<double> pk_gc, pk_gf[3], pk_kr;
+  bool sym = pack_geo(g, gcon, G.ncell, pk_gc);
+  const double *gbs[3] = {gb0, gb1, gb2}, *gcbs[3] = {gconb0, gconb1, gconb2};
+  for (int d = 0; d < 3; ++d) sym = sym && pack_geo(gbs[d], gcbs[d], G.nface[d], pk_gf[d]);
+  sym = sym && pack_kris(kris, G.ncell, pk_kr);
+  if (!sym) {  // MetricCache builds g and the Christoffels symmetric bit for bit
+    *err = KZ_E_INVALID_CONFIG;
+    delete s;
+    return nullptr;
+  }
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { *err = KZ_E_HIP; delete s; return nullptr; }
   size_t fb = (size_t)G.ncell * NV * sizeof(double);
   Dev &D = s->d;
@@ -1121,7 +1767,7 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   for (int c = 0; c < 3; ++c) e |= dalloc(s, (void **)&D.emf[c], ncorn * sizeof(double));
   e |= dalloc(s, (void **)&D.gc, (size_t)G.ncell * GEO_NC * sizeof(double));
   for (int d = 0; d < 3; ++d) e |= dalloc(s, (void **)&D.gf[d], (size_t)G.nface[d] * GEO_NC * sizeof(double));
-  e |= dalloc(s, (void **)&D.kris, (size_t)G.ncell * 64 * sizeof(double));
+  e |= dalloc(s, (void **)&D.kris, (size_t)G.ncell * KR_NC * sizeof(double));
   size_t nbcx = (size_t)gr->nz * gr->ny * gr->ng * BCX_REC;
   e |= dalloc(s, (void **)&D.bcx, nbcx * sizeof(double));
   e |= dalloc(s, (void **)&D.cnt, 8 * sizeof(unsigned long long));
@@ -1134,20 +1780,20 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   e |= dalloc(s, (void **)&D.imp_aux, (size_t)nd * IMP_NAUX * sizeof(double));
   e |= dalloc(s, (void **)&D.imp_slow, (size_t)nd * sizeof(int32_t));
   e |= dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long));
+  e |= dalloc(s, (void **)&D.sync, SY_N * sizeof(unsigned));
+  e |= dalloc(s, (void **)&D.fix_list, (size_t)nd * sizeof(int32_t));
+  if (ph->radviscosity || ph->dynamo) e |= dalloc(s, (void **)&D.bl, (size_t)G.ncell * BL_NC * sizeof(double));
+  if (ph->radviscosity) e |= dalloc(s, (void **)&D.rijvisc, (size_t)G.ncell * 16 * sizeof(double));
+  if (ph->dynamo) {
+    e |= dalloc(s, (void **)&D.dyn_a, (size_t)G.ncell * sizeof(double));
+    e |= dalloc(s, (void **)&D.scaleth, (size_t)gr->nx * sizeof(double));
+  }
   if (e) { *err = e < 0 ? e : KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // metric tables
-  std::vector<double> buf;
-  pack_geo(g, gcon, G.ncell, buf);
-  e |= hipMemcpy(D.gc, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
-  const double *gbs[3] = {gb0, gb1, gb2}, *gcbs[3] = {gconb0, gconb1, gconb2};
-  for (int d = 0; d < 3; ++d) {
-    pack_geo(gbs[d], gcbs[d], G.nface[d], buf);
-    e |= hipMemcpy(D.gf[d], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
-  }
-  buf.assign((size_t)G.ncell * 64, 0.0);
-  for (long c = 0; c < G.ncell; ++c)
-    for (int k = 0; k < 64; ++k) buf[(size_t)k * G.ncell + c] = kris[c * 64 + k];
-  e |= hipMemcpy(D.kris, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+  e |= hipMemcpy(D.gc, pk_gc.data(), pk_gc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+  for (int d = 0; d < 3; ++d)
+    e |= hipMemcpy(D.gf[d], pk_gf[d].data(), pk_gf[d].size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+  e |= hipMemcpy(D.kris, pk_kr.data(), pk_kr.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
   e |= hipMemcpy(D.bcx, bcx, nbcx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
   if (e) { *err = KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // set_grid minimum cell sizes incl. the mdx quirk (sim.zig:447-466)
@@ -1163,6 +1809,13 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
         if (dzv < mdx || mdz < 0) mdz = dzv;
       }
   s->min_dx = mdx; s->min_dy = mdy; s->min_dz = mdz;
+  {  // metric.zig:20-32 rHorizonBL / rIscoBL (pow(., 1/2) taken as a square root)
+    const double a = ph->bh_spin, c3 = 1.0 / 3.0;
+    s->rhor = 1.0 + kzm_sqrt(1.0 - a * a);
+    double z1 = 1.0 + kzm_pow(1.0 - a * a, c3) * (kzm_pow(1.0 + a, c3) + kzm_pow(1.0 - a, c3));
+    double z2 = kzm_sqrt(3.0 * a * a + z1 * z1);
+    s->risco = 3.0 + z2 - kzm_sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+  }
   s->t_host = 0;
   s->nstep = 0;
   unsigned long long tsd0[2] = {(unsigned long long)__builtin_bit_cast(long long, 0.0),
@@ -1246,8 +1899,8 @@ static void launch_set_bc(kz_sim *s) {
   const GridD &g = s->g;
   TimedLaunch tl(s, KZ_K_BC);
   long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2;
-  k_bc_xy<<<nblk(nb, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
-  k_bc_corners2d<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
+  k_bc<<<nblk(nb, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
+  if (g.nz == 1) k_bc_corners<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
 }
 
 int kz_sim_set_bc(kz_sim *s, int ifinit) {
@@ -1256,43 +1909,70 @@ int kz_sim_set_bc(kz_sim *s, int ifinit) {
   return check_err(s);
 }
 
-static void launch_calc_u2p(kz_sim *s) {
-  long n = ndom(s);
+// calcU2p (sim.zig:924-1026): u2p + fixups, then setBc when `with_bc`.
+// Inside a step every calcU2p is followed either by doCorrect + setBc or by
+// passes that read no ghost cell before the next setBc (the implicit solve,
+// fixups, polar overwrite and stage arithmetic read domain cells only), so
+// the step omits that setBc: it would write ghosts that are overwritten
+// before they are read.
+static void launch_calc_u2p(kz_sim *s, bool with_bc) {
   {
     TimedLaunch tl(s, KZ_K_U2P);
-    k_u2p<<<nblk(n, 128), 128, 0, s->stream>>>(s->g, s->d, s->ph);
+    k_u2p<<<nblk(ndom(s), 128), 128, 0, s->stream>>>(s->g, s->d, s->ph);
   }
-  if (s->ph.do_fixups) k_fixup<<<nblk(n, 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_HD_FIXUP);
-  if (s->ph.do_fixups && s->ph.do_u2prad_fixups)
-    k_fixup<<<nblk(n, 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RAD_FIXUP);
-  launch_set_bc(s);
+  if (s->ph.do_fixups) k_u2p_fixups<<<1, 256, 0, s->stream>>>(s->g, s->d, s->ph);
+  if (with_bc) launch_set_bc(s);
 }
 
-static void launch_op_explicit(kz_sim *s, int dt_slot) {
+static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
   const GridD &g = s->g;
   long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
   {
     TimedLaunch tl(s, KZ_K_WAVESPEEDS);
     k_wavespeeds<<<nblk(nw, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
-  for (int dim = 0; dim < 2; ++dim) {
-    long nd = dim == 0 ? g.nx : g.ny;
-    long w0 = dim == 0 ? g.ny + 2 : g.nx + 2;
-    long nf = (nd + 1) * w0 * g.nz;
+  {
+    long nf = (long)(g.nx + 1) * (g.ny + 2) * g.nz + (long)(g.ny + 1) * (g.nx + 2) * g.nz;
     TimedLaunch tl(s, KZ_K_FACE_FLUX);
-    k_face_flux<<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph, dim);
+    if (s->ph.radviscosity) k_face_flux<true><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
+    else k_face_flux<false><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
   long nc = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   {
     TimedLaunch tl(s, KZ_K_CT);
     k_emf<<<nblk(nc, 256), 256, 0, s->stream>>>(g, s->d);
-    k_ct_rebuild<<<nblk(nc, 256), 256, 0, s->stream>>>(g, s->d);
   }
   {
     TimedLaunch tl(s, KZ_K_UPDATE);
     k_update<<<nblk(ndom(s), 128), 128, 0, s->stream>>>(g, s->d, s->ph, dt_slot);
   }
-  launch_calc_u2p(s);
+  launch_calc_u2p(s, with_bc);
+}
+
+// dynamo.zig:462-469 applyDynamo: scale height, setBc, mimicDynamo,
+// calcU2p (its setBc only when `with_bc`, as for launch_calc_u2p).
+static void launch_dynamo(kz_sim *s, int dt_slot, bool with_bc) {
+  const GridD &g = s->g;
+  {
+    TimedLaunch tl(s, KZ_K_DYNAMO);
+    k_scale_height<<<nblk(g.nx, 64), 64, 0, s->stream>>>(g, s->d);
+  }
+  launch_set_bc(s);
+  {
+    TimedLaunch tl(s, KZ_K_DYNAMO);
+    long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
+    k_dynamo_a<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor, s->risco);
+    k_dynamo_curl<<<nblk((long)g.nx * g.ny, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
+  }
+  launch_calc_u2p(s, with_bc);
+}
+
+// rijvisc.zig:266-311 calcRijViscTotal over the domain and its ring
+static void launch_rijvisc(kz_sim *s, int dt_slot) {
+  const GridD &g = s->g;
+  TimedLaunch tl(s, KZ_K_VISC);
+  long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
+  k_rijvisc<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor);
 }
 
 static void launch_op_implicit(kz_sim *s, int dt_slot) {
@@ -1316,7 +1996,7 @@ static void launch_do_correct(kz_sim *s) {
 }
 
 int kz_calc_u2p(kz_sim *s) {
-  launch_calc_u2p(s);
+  launch_calc_u2p(s, true);
   return check_err(s);
 }
 
@@ -1329,7 +2009,7 @@ static int put_dt(kz_sim *s, double dt) {
 
 int kz_op_explicit(kz_sim *s, double dt) {
   if (int e = put_dt(s, dt)) return e;
-  launch_op_explicit(s, SS_USER);
+  launch_op_explicit(s, SS_USER, true);
   return check_err(s);
 }
 
@@ -1344,50 +2024,80 @@ int kz_do_correct(kz_sim *s) {
   return check_err(s);
 }
 
+int kz_sim_set_bl_tables(kz_sim *s, const double *bl) {
+  if (!s->d.bl) return KZ_E_INVALID_CONFIG;
+  const long n = s->g.ncell;
+  std::vector<double> planes((size_t)n * BL_NC);
+  for (long c = 0; c < n; ++c)
+    for (int k = 0; k < BL_NC; ++k) planes[(size_t)k * n + c] = bl[(size_t)c * BL_NC + k];
+  HIPCHK(hipMemcpyAsync(s->d.bl, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->has_bl = true;
+  return 0;
+}
+
+int kz_calc_rij_visc_total(kz_sim *s, double global_dt) {
+  if (!s->d.rijvisc || !s->has_bl) return KZ_E_INVALID_CONFIG;
+  if (int e = put_dt(s, global_dt)) return e;
+  launch_rijvisc(s, SS_USER);
+  return check_err(s);
+}
+
+int kz_sim_get_rijvisc(kz_sim *s, double *out) {
+  if (!s->d.rijvisc) return KZ_E_INVALID_CONFIG;
+  const long n = s->g.ncell;
+  std::vector<double> planes((size_t)n * 16);
+  HIPCHK(hipMemcpyAsync(planes.data(), s->d.rijvisc, planes.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (long c = 0; c < n; ++c)
+    for (int k = 0; k < 16; ++k) out[(size_t)c * 16 + k] = planes[(size_t)k * n + c];
+  return 0;
+}
+
+int kz_apply_dynamo(kz_sim *s, double dt) {
+  if (!s->d.dyn_a || !s->has_bl) return KZ_E_INVALID_CONFIG;
+  if (int e = put_dt(s, dt)) return e;
+  launch_dynamo(s, SS_USER, true);
+  return check_err(s);
+}
+
 // Enqueue one RK2IMEX step (sim.zig:1792-1899) on the sim's stream; no host
-// synchronisation inside.
+// synchronisation inside.  saveTimesteps (cell_dt) is omitted: nothing on
+// this path reads it.
 static void launch_step(kz_sim *s, double forced, int use_forced) {
   const GridD &g = s->g;
   Dev &d = s->d;
   hipStream_t st = s->stream;
-  long n = ndom(s);
-  long nn = n * NV;
-  size_t fb = (size_t)g.ncell * NV * sizeof(double);
-  k_begin_step<<<1, 1, 0, st>>>(d, forced, use_forced);
-  // saveTimesteps writes cell_dt, which nothing on this path reads.
-  (void)hipMemcpyAsync(d.ut0, d.u, fb, hipMemcpyDeviceToDevice, st);
-  launch_op_implicit(s, SS_DTG);
-  k_stage_deriv<<<nblk(nn, 256), 256, 0, st>>>(g, d.drt1, d.u, d.ut0, d.ss, SS_INV_DTG, SS_NINV_DTG);
-  (void)hipMemcpyAsync(d.ut1, d.u, fb, hipMemcpyDeviceToDevice, st);
-  launch_calc_u2p(s);
+  const unsigned nb = nblk(ndom(s), 256);
+  k_begin_step<<<nb, 256, 0, st>>>(g, d, forced, use_forced);  // + ut0 = u
+  if (s->ph.radviscosity) launch_rijvisc(s, SS_DT);             // sim.zig:1819-1823
+  launch_op_implicit(s, SS_DTG);                                // U(1)
+  k_stage_implicit<<<nb, 256, 0, st>>>(g, d, d.drt1, d.ut0, d.ut1);
+  launch_calc_u2p(s, false);
   launch_do_correct(s);
   launch_set_bc(s);
-  launch_op_explicit(s, SS_DT);
-  k_copy_entropy_flag<<<nblk(n, 256), 256, 0, st>>>(g, d);
-  k_stage_deriv<<<nblk(nn, 256), 256, 0, st>>>(g, d.dut1, d.u, d.ut1, d.ss, SS_INV_DT, SS_NINV_DT);
-  // u = ut0 + dt*dut1 + dt(1-2g)*drt1
-  k_stage_combine<<<nblk(nn, 256), 256, 0, st>>>(g, d.u, d.ut0, d.dut1, d.drt1, d.ss, SS_DT, SS_C12);
-  (void)hipMemcpyAsync(d.uforget, d.u, fb, hipMemcpyDeviceToDevice, st);
-  launch_calc_u2p(s);
+  launch_op_explicit(s, SS_DT, false);                          // F(U(1))
+  if (s->ph.dynamo) launch_dynamo(s, SS_DT, false);             // sim.zig:1845-1849
+  k_stage_explicit1<<<nb, 256, 0, st>>>(g, d);
+  launch_calc_u2p(s, false);
   launch_do_correct(s);
-  launch_op_implicit(s, SS_DTG);
-  k_stage_deriv<<<nblk(nn, 256), 256, 0, st>>>(g, d.drt2, d.u, d.uforget, d.ss, SS_INV_DTG, SS_NINV_DTG);
-  (void)hipMemcpyAsync(d.ut2, d.u, fb, hipMemcpyDeviceToDevice, st);
-  launch_calc_u2p(s);
+  launch_op_implicit(s, SS_DTG);                                // U(2)
+  k_stage_implicit<<<nb, 256, 0, st>>>(g, d, d.drt2, d.uforget, d.ut2);
+  launch_calc_u2p(s, false);
   launch_do_correct(s);
   launch_set_bc(s);
-  launch_op_explicit(s, SS_DT);
-  k_stage_deriv<<<nblk(nn, 256), 256, 0, st>>>(g, d.dut2, d.u, d.ut2, d.ss, SS_INV_DT, SS_NINV_DT);
-  k_stage_combine<<<nblk(nn, 256), 256, 0, st>>>(g, d.u, d.ut0, d.dut1, d.dut2, d.ss, SS_HALF, SS_HALF);
-  k_stage_combine<<<nblk(nn, 256), 256, 0, st>>>(g, d.u, d.u, d.drt1, d.drt2, d.ss, SS_HALF, SS_HALF);
-  launch_calc_u2p(s);
+  launch_op_explicit(s, SS_DT, false);                          // F(U(2))
+  if (s->ph.dynamo) launch_dynamo(s, SS_DT, false);             // sim.zig:1872-1876
+  k_stage_explicit2<<<nb, 256, 0, st>>>(g, d);
+  launch_calc_u2p(s, false);
   launch_do_correct(s);
   launch_set_bc(s);
-  k_end_step<<<1, 1, 0, st>>>(d);
-  k_update_entropy<<<nblk(n, 256), 256, 0, st>>>(g, d, s->ph);
+  k_end_step<<<nb, 256, 0, st>>>(g, d, s->ph);                  // t += dt, updateEntropy
 }
 
 int kz_step(kz_sim *s, const double *forced_dt) {
+  if ((s->ph.radviscosity || s->ph.dynamo) && !s->has_bl) return KZ_E_INVALID_CONFIG;
   launch_step(s, forced_dt ? *forced_dt : 0.0, forced_dt != nullptr);
   s->nstep += 1;
   return check_err(s);
@@ -1395,6 +2105,7 @@ int kz_step(kz_sim *s, const double *forced_dt) {
 
 // enqueue `nsteps` CFL steps without host synchronisation (benchmarks)
 int kz_step_many(kz_sim *s, int nsteps) {
+  if ((s->ph.radviscosity || s->ph.dynamo) && !s->has_bl) return KZ_E_INVALID_CONFIG;
   for (int i = 0; i < nsteps; ++i) {
     launch_step(s, 0.0, 0);
     s->nstep += 1;

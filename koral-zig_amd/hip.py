@@ -43,6 +43,10 @@ _SIGS = {
     "kz_sim_counters": (C.c_int, [C.c_void_p, C.POINTER(T.kz_counters)]),
     "kz_sim_time_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_kernel_time": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_uint64)]),
+    "kz_sim_set_bl_tables": (C.c_int, [C.c_void_p, D]),
+    "kz_calc_rij_visc_total": (C.c_int, [C.c_void_p, C.c_double]),
+    "kz_sim_get_rijvisc": (C.c_int, [C.c_void_p, D]),
+    "kz_apply_dynamo": (C.c_int, [C.c_void_p, C.c_double]),
     # kz_cellops.h
     "kz_p2u_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, D]),
     "kz_u2p_mhd_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.c_double, C.POINTER(T.kz_floor_params), C.c_void_p]),
@@ -193,6 +197,21 @@ class Sim:
         k = -1 if kernel is None else KERNELS[kernel]
         _chk(lib().kz_sim_time_kernel(self.h, k), "kz_sim_time_kernel")
 
+    def set_bl_tables(self, bl):
+        """Load the BL sidecar (host.bl_tables) for radiative viscosity / dynamo."""
+        bl = _f64(bl)
+        _chk(lib().kz_sim_set_bl_tables(self.h, T.ptr(bl)), "kz_sim_set_bl_tables")
+
+    def calc_rij_visc_total(self, global_dt):
+        """Viscous R^i_j for step size global_dt; returns it, 16 per padded cell."""
+        _chk(lib().kz_calc_rij_visc_total(self.h, global_dt), "kz_calc_rij_visc_total")
+        r = np.zeros(self.shape + (16,))
+        _chk(lib().kz_sim_get_rijvisc(self.h, T.ptr(r)), "kz_sim_get_rijvisc")
+        return r
+
+    def apply_dynamo(self, dt):
+        _chk(lib().kz_apply_dynamo(self.h, dt), "kz_apply_dynamo")
+
     def kernel_time(self):
         """(summed ms, launches) of the timed class since the last call."""
         ms, n = C.c_double(0.0), C.c_uint64(0)
@@ -200,7 +219,8 @@ class Sim:
         return ms.value, n.value
 
 
-KERNELS = {"wavespeeds": 0, "face_flux": 1, "ct": 2, "update": 3, "u2p": 4, "implicit": 5, "bc": 6}
+KERNELS = {"wavespeeds": 0, "face_flux": 1, "ct": 2, "update": 3, "u2p": 4, "implicit": 5, "bc": 6, "visc": 7,
+           "dynamo": 8}
 
 
 # ---- per-point batches (include/kz_cellops.h) ------------------------------

@@ -44,13 +44,23 @@ def test_struct_sizes():
     assert C.sizeof(T.kz_grid) == 4 * 4 + 6 * 8
 
 
-@pytest.mark.parametrize("field", ["radviscosity", "dynamo", "reduceorderatbh", "reduceorderafterfixup"])
+@pytest.mark.parametrize("field", ["reduceorderatbh", "reduceorderafterfixup"])
 def test_create_rejects_unsupported(field):
     grid = T.grid_puffy(16, 12)
     tab = host.metric_tables(grid)
     phys = T.physics_puffy()
     setattr(phys, field, 1)
     with pytest.raises(RuntimeError, match="Unsupported|-7"):
+        hip.Sim(grid, phys, tab)
+
+
+def test_create_rejects_radvisc_without_opacities():
+    """sim.zig:319-363: radiative viscosity needs the opacities."""
+    grid = T.grid_puffy(16, 12)
+    tab = host.metric_tables(grid)
+    phys = T.physics_puffy(radviscosity=True)
+    phys.has_opac = 0
+    with pytest.raises(RuntimeError, match="InvalidConfig"):
         hip.Sim(grid, phys, tab)
 
 

@@ -157,12 +157,3 @@ def test_set_time_resume():
     gpu.close()
     gpu2.close()
 
-
-def test_unsupported_options_rejected():
-    grid = T.grid_puffy(16, 12)
-    tab = host.metric_tables(grid)
-    for field in ("radviscosity", "dynamo"):
-        phys = T.physics_puffy()
-        setattr(phys, field, 1)
-        with pytest.raises(RuntimeError, match="kz_sim_create"):
-            hip.Sim(grid, phys, tab)
