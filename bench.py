@@ -46,6 +46,11 @@ BYTES_PER_UNIT = {"face_flux": 400.0, "update": 1224.0, "u2p": 504.0}
 # count later.
 IMPLICIT_FLOPS_PER_SOLVE = 12000.0
 IMPLICIT_FLOPS_PER_ITER = 10000.0
+# HBM bytes per launch of a kernel class at 384x360, from rocprofv3 runs of
+# FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
+# counter reads for a copy kernel of known size in the same run (k_soa2aos:
+# fetch x1.132, write x1.000).  Only valid for the default 384x360 workload.
+TRAFFIC_BYTES_PER_LAUNCH_C1 = {"implicit": 2.04e8, "update": 1.33e8, "face_flux": 1.27e8, "u2p": 6.28e7}
 
 
 def _pkg():
@@ -78,6 +83,65 @@ def cpu_baseline(grid, phys, tab, p, u, clock, steps):
     return grid.nx * grid.ny * steps / dt, dt
 
 
+def bench_implicit_batch(args):
+    """The independent-cell implicit-solve batch (2^22 cells by default):
+    cell-solves/s of the device-resident compacted solve, inputs resident in
+    HBM; the oracle times a bounded sample of the same cells."""
+    from koral_zig_amd import abi_types as T
+    from koral_zig_amd import hip, workloads
+
+    phys = T.physics_puffy()
+    n = args.cells
+    t0 = time.perf_counter()
+    pp, geom, dt = workloads.implicit_batch_inputs(
+        n, args.seed, phys, lambda p, g: hip.radstate(p, g, phys.gam, phys.opac, 0)[:, 9])
+    b = hip.ImplicitBatch(n, phys)
+    b.set(pp, geom, dt)
+    log(f"setup {time.perf_counter() - t0:.1f}s ({n} cells, seed {args.seed:#x})")
+    for _ in range(max(args.warmup, 1)):
+        b.solve()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(b.solve())
+    wall = time.perf_counter() - t0
+    _, _, res, (nfail, iters, solves) = b.get()
+    b.close()
+    value = n * args.steps / wall
+    avg_ms = float(np.mean(kms))
+    flops = IMPLICIT_FLOPS_PER_SOLVE * n + IMPLICIT_FLOPS_PER_ITER * iters
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    log(f"timed {args.steps} solves in {wall:.3f}s: {value:.4e} cell-solves/s; kernels {avg_ms:.3f} ms/solve; "
+        f"ok {solves}/{n}, {iters / max(solves, 1):.2f} iterations per solve")
+    cpu = None
+    if args.cpu_cells > 0:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import kzo
+        m = min(args.cpu_cells, n)
+        t1 = time.perf_counter()
+        kzo.solve_implicit(pp[:m], geom[:m], dt[:m], phys.gam, phys.rad, phys.opac, phys.implicit)
+        secs = time.perf_counter() - t1
+        cpu = {"value": m / secs, "unit": "cell-solves/s", "cores": 1, "kind": "port",
+               "sample": f"the first {m} cells of the same batch ({secs:.1f}s, one thread)"}
+        log(f"cpu baseline {m / secs:.4e} cell-solves/s over {secs:.1f}s")
+    out = {
+        "metric": "cell-solves/s implicit radiation-gas four-force batch on 1 GPU",
+        "value": value, "unit": "cell-solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"independent-cell implicit batch, {n} cells (rho, T_gas, E_hat/u, |F|/E, tau, "
+                               f"MKS2 a=0 r in [2.5, 50] spread as specified in koral_zig_amd.workloads)",
+                   "cells": n, "seed": args.seed},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "implicit",
+                     "avg_launch_ms": avg_ms, "launches": len(kms)},
+        "cpu_baseline": cpu,
+        "solver": {"failures": nfail, "iterations": iters, "solves": solves,
+                   "rungs": np.bincount(res["rung"][res["ok"] == 1], minlength=6).tolist()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +152,10 @@ def main():
     ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--latency-steps", type=int, default=50)
+    ap.add_argument("--workload", choices=["puffy", "implicit-batch"], default="puffy",
+                    help="puffy: the 384x360 step (default); implicit-batch: the independent-cell solve batch")
+    ap.add_argument("--cells", type=int, default=1 << 22, help="implicit-batch: number of cells")
+    ap.add_argument("--cpu-cells", type=int, default=20000, help="implicit-batch: oracle sample size (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs the step on one GPU only; --gpus must be 1 "
@@ -95,6 +163,13 @@ def main():
         sys.exit(2)
 
     _pkg()
+    if args.workload == "implicit-batch":
+        if "--steps" not in sys.argv:
+            args.steps = 5
+        if "--warmup" not in sys.argv:
+            args.warmup = 1
+        bench_implicit_batch(args)
+        return
     from koral_zig_amd import abi_types as T
     from koral_zig_amd import hip, host
 
@@ -153,7 +228,7 @@ def main():
         achieved = flops / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = (TRAFFIC_BYTES_PER_LAUNCH_C1.get(dom) if (args.nx, args.ny) == (384, 360) else None)
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_ms
     roof["launches"] = n_dom

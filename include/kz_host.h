@@ -27,6 +27,10 @@ int kz_host_metric_tables(const kz_grid *g, double a, double mksr0, double mksh0
 /* metric.compute at one point (koral/metric/metric.zig:64-104);
  * coords 0 Minkowski, 1 BL, 2 KS, 3 MKS2; out: gg16 GG16 gdet dlgdet3 Kr64 gttpert */
 int kz_host_metric_point(int coords, double a, double mksr0, double mksh0, const double *x4, double *out101);
+/* kz_geom records (MetricCache.fillGeometry, koral/metric/precompute.zig:
+ * 499-507) at n points x4[4 n] of coordinate system `coords`. */
+int kz_host_geom_batch(int coords, double a, double mksr0, double mksh0, int64_t n, const double *x4,
+                       kz_geom *out);
 
 /* coco.dxdx (koral/metric/coco.zig:144-160): dir 0 MKS2->BL, 1 BL->MKS2 */
 int kz_host_dxdx(int dir, double a, double mksr0, double mksh0, const double *x4, double *out16);

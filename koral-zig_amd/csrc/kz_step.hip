@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/kz_step.h"
+#include "../../include/kz_implicit.h"
 #include "kz_dev.h"
 #include "kz_dev_solve.h"
 
@@ -64,6 +65,8 @@ struct Dev {
   double *imp_aux;   // [IMP_NAUX][nd] reference-state scalars, see ImpAux
   int32_t *imp_slow;           // cells that need rungs past the first
   unsigned long long *imp_nslow;
+  int32_t *imp_list;             // [2][nd] cells by starting block (MHD, RAD)
+  unsigned long long *imp_ncnt;  // [2] lengths of the two lists
   // the u2p fixup list
   unsigned *sync;     // [SY_NFIX] u2p fixup list length
   int32_t *fix_list;  // domain cells flagged for a u2p fixup
@@ -150,22 +153,25 @@ __device__ __forceinline__ bool polar_row(const kz_physics &ph, const GridD &g, 
 #define TID (blockIdx.x * (long)blockDim.x + threadIdx.x)
 
 // Minimum waves per SIMD each heavy kernel is compiled for (1 = no bound;
-// n bounds the kernel to 512/n vector registers).  Tuned by measurement.
+// n bounds the kernel to 512/n vector registers).  Chosen by a measured
+// sweep at 384x360: 2 waves help the memory-bound update and u2p passes and
+// the implicit preparation (their register need is just above 256); the face
+// flux and the first Newton rung lose more to spills than they gain.
 #define KZ_WPE(n) __attribute__((amdgpu_waves_per_eu(n)))
 #ifndef KZ_WPE_FACE
 #define KZ_WPE_FACE 1
 #endif
 #ifndef KZ_WPE_UPDATE
-#define KZ_WPE_UPDATE 1
+#define KZ_WPE_UPDATE 2
 #endif
 #ifndef KZ_WPE_U2P
-#define KZ_WPE_U2P 1
+#define KZ_WPE_U2P 2
 #endif
 #ifndef KZ_WPE_WS
 #define KZ_WPE_WS 1
 #endif
 #ifndef KZ_WPE_PREP
-#define KZ_WPE_PREP 1
+#define KZ_WPE_PREP 2
 #endif
 #ifndef KZ_WPE_RUNG0
 #define KZ_WPE_RUNG0 1
@@ -891,23 +897,102 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, D
 // The sequence of attempts per cell is the reference ladder's; inputs to
 // every rung are the stored reference-state values, so results equal the
 // single-pass ladder bit for bit.
-__device__ __forceinline__ long dom_t(const GridD &g, long ix, long iy, long iz) {
-  return ix + (long)g.nx * (iy + (long)g.ny * iz);
-}
-__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(GridD g, Dev d, kz_physics ph, int sslot) {
+// Scratch of the split solve, indexed by the solve's cell index t.
+struct ImpWork {
+  double *uu00;  // [13][n] conserved state of the incoming primitives
+  double *pp0;   // [13][n] Newton start (after the bisection warm start)
+  double *aux;   // [IMP_NAUX][n] reference-state scalars
+  int32_t *slow;                  // cells that need rungs past the first
+  int32_t *list;                  // [2][n] cells by starting block (MHD, RAD)
+  unsigned long long *nslow;      // length of `slow`
+  unsigned long long *ncnt;       // [2] lengths of the two lists
+};
+// The cells of one run's domain (opImplicit, sim.zig:1736-1776): polar rows
+// are skipped, dt is a step scalar, a solved cell stores u and p and clears
+// its radimp flag (sim.zig:1762-1768), a failed one sets it to -1.
+struct GridCells {
+  GridD g;
+  Dev d;
+  int sslot;
+  __device__ long count() const { return (long)g.nx * g.ny * g.nz; }
+  __device__ void ijk(long t, long &ix, long &iy, long &iz) const {
+    ix = t % g.nx;
+    iy = (t / g.nx) % g.ny;
+    iz = t / ((long)g.nx * g.ny);
+  }
+  __device__ bool skip(const kz_physics &ph, long t) const { return polar_row(ph, g, (t / g.nx) % g.ny); }
+  __device__ void load_p(long t, double *pp) const {
+    long ix, iy, iz;
+    ijk(t, ix, iy, iz);
+    ld(d.p, g.ncell, cidx(g, ix, iy, iz), pp);
+  }
+  __device__ void geo(long t, Geo &gm) const {
+    long ix, iy, iz;
+    ijk(t, ix, iy, iz);
+    geo_c(g, d, ix, iy, iz, gm);
+  }
+  __device__ double dt(long) const { return d.ss[sslot]; }
+  __device__ void solved(const kz_physics &ph, long t, const Geo &gm, const double *ppw, int iters, int, int, int,
+                         int) const {
+    long ix, iy, iz;
+    ijk(t, ix, iy, iz);
+    const long c = cidx(g, ix, iy, iz);
+    double uu[NV];
+    p2u(ppw, uu, gm, ph.gam);
+    st(d.u, g.ncell, c, uu);
+    st(d.p, g.ncell, c, ppw);
+    d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + c] = 0;
+    atomicAdd(&d.cnt[1], (unsigned long long)iters);
+    atomicAdd(&d.cnt[2], 1ULL);
+  }
+  __device__ void failed(long t) const {
+    long ix, iy, iz;
+    ijk(t, ix, iy, iz);
+    d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + cidx(g, ix, iy, iz)] = -1;
+    atomicAdd(&d.cnt[0], 1ULL);
+  }
+};
+// n independent cells (solveImplicitLab per record, implicit.zig:927-1009):
+// geometry planes and dt per cell, the ladder's outcome recorded per cell.
+struct BatchCells {
+  long n;
+  const double *gp;   // [GEO_NC + 1][n]: the geometry planes, then x1
+  const double *dtv;  // [n]
+  double *uu, *pp;    // [NV][n]
+  kz_implicit_result *res;
+  unsigned long long *cnt;  // [0] failures [1] iterations [2] solves
+  __device__ long count() const { return n; }
+  __device__ bool skip(const kz_physics &, long) const { return false; }
+  __device__ void load_p(long t, double *p) const { ld(pp, n, t, p); }
+  __device__ void geo(long t, Geo &gm) const { load_geo(gp, n, t, gp[GEO_NC * n + t], gm); }
+  __device__ double dt(long t) const { return dtv[t]; }
+  __device__ void solved(const kz_physics &ph, long t, const Geo &gm, const double *ppw, int iters, int ir, int wp,
+                         int we, int wf) const {
+    double u[NV];
+    p2u(ppw, u, gm, ph.gam);
+    st(uu, n, t, u);
+    st(pp, n, t, ppw);
+    res[t] = kz_implicit_result{1, ir, iters, wp, we, wf};
+    atomicAdd(&cnt[1], (unsigned long long)iters);
+    atomicAdd(&cnt[2], 1ULL);
+  }
+  __device__ void failed(long t) const {
+    res[t] = kz_implicit_result{0, -1, 0, WP_RAD, WE_ENERGY, WF_LAB};
+    atomicAdd(&cnt[0], 1ULL);
+  }
+};
+template <class S>
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(S s, ImpWork w, kz_physics ph) {
   long t = TID;
-  if (t == 0) *d.imp_nslow = 0;  // no appends run concurrently with this kernel
-  long nd = (long)g.nx * g.ny * g.nz;
+  if (t == 0) *w.nslow = 0;  // no appends run concurrently with this kernel
+  const long nd = s.count();
   if (t >= nd) return;
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  if (polar_row(ph, g, iy)) return;
-  long n = g.ncell;
-  long c = cidx(g, ix, iy, iz);
-  double dt = d.ss[sslot];
+  if (s.skip(ph, t)) return;
+  double dt = s.dt(t);
   double pp00[NV], uu00[NV], pp0[NV];
-  ld(d.p, n, c, pp00);
+  s.load_p(t, pp00);
   Geo gm;
-  geo_c(g, d, ix, iy, iz, gm);
+  s.geo(t, gm);
   p2u(pp00, uu00, gm, ph.gam);
   double ehat = -calc_ff_rtt(pp00, gm);
   RadState st00;
@@ -920,9 +1005,18 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(GridD g, De
     for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
   }
   int sw = (ehat < ph.implicit.threshold * pp00[UU]) ? WP_RAD : WP_MHD;
-  st(d.imp_uu00, nd, t, uu00);
-  st(d.imp_pp0, nd, t, pp0);
-  double *a = d.imp_aux;
+  // gather the cell into the list of its starting block, so that each
+  // first-rung kernel runs full waves of one Newton variant
+  if (sw == WP_MHD) {
+    unsigned long long k = atomicAdd(&w.ncnt[0], 1ULL);
+    w.list[k] = (int32_t)t;
+  } else {
+    unsigned long long k = atomicAdd(&w.ncnt[1], 1ULL);
+    w.list[nd + k] = (int32_t)t;
+  }
+  st(w.uu00, nd, t, uu00);
+  st(w.pp0, nd, t, pp0);
+  double *a = w.aux;
   a[IA_EHAT * nd + t] = st00.ehat;
   a[IA_UINT * nd + t] = st00.uint;
   a[IA_SGAS * nd + t] = st00.sgas;
@@ -932,11 +1026,11 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(GridD g, De
   a[IA_KRR * nd + t] = st00.op.rad_ross;
   a[IA_SW * nd + t] = (double)sw;
 }
-__device__ __forceinline__ void imp_load(const Dev &d, long nd, long t, double *uu00, double *pp0, State0 &s0,
+__device__ __forceinline__ void imp_load(const ImpWork &w, long nd, long t, double *uu00, double *pp0, State0 &s0,
                                          LagOpac &lag) {
-  ld(d.imp_uu00, nd, t, uu00);
-  ld(d.imp_pp0, nd, t, pp0);
-  const double *a = d.imp_aux;
+  ld(w.uu00, nd, t, uu00);
+  ld(w.pp0, nd, t, pp0);
+  const double *a = w.aux;
   s0.ehat = a[IA_EHAT * nd + t];
   s0.uint = a[IA_UINT * nd + t];
   s0.sgas = a[IA_SGAS * nd + t];
@@ -945,59 +1039,48 @@ __device__ __forceinline__ void imp_load(const Dev &d, long nd, long t, double *
   lag.op.rad_abs = a[IA_KRA * nd + t];
   lag.op.rad_ross = a[IA_KRR * nd + t];
 }
-// store a solved cell (sim.zig:1762-1768)
-__device__ __forceinline__ void imp_commit(const GridD &g, const Dev &d, const kz_physics &ph, long c, const Geo &gm,
-                                           const double *ppw, int iters) {
-  double uu[NV];
-  p2u(ppw, uu, gm, ph.gam);
-  st(d.u, g.ncell, c, uu);
-  st(d.p, g.ncell, c, ppw);
-  d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + c] = 0;
-  atomicAdd(&d.cnt[1], (unsigned long long)iters);
-  atomicAdd(&d.cnt[2], 1ULL);
-}
-template <int WP>
-__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(GridD g, Dev d, kz_physics ph, int sslot) {
-  long t = TID;
-  long nd = (long)g.nx * g.ny * g.nz;
-  if (t >= nd) return;
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  if (polar_row(ph, g, iy)) return;
-  if ((int)d.imp_aux[IA_SW * nd + t] != WP) return;
+template <class S, int WP>
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(S s, ImpWork w, kz_physics ph) {
+  constexpr int which = WP == WP_MHD ? 0 : 1;
+  long i = TID;
+  if (i >= (long)w.ncnt[which]) return;
+  const long nd = s.count();
+  long t = w.list[which * nd + i];
   double uu00[NV], pp0[NV], ppw[NV];
   State0 s0;
   LagOpac lag;
-  imp_load(d, nd, t, uu00, pp0, s0, lag);
+  imp_load(w, nd, t, uu00, pp0, s0, lag);
   Geo gm;
-  geo_c(g, d, ix, iy, iz, gm);
-  double dt = d.ss[sslot];
+  s.geo(t, gm);
+  double dt = s.dt(t);
   ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
   int iters = 0;
   bool ok = ph.implicit.opdamp_maxlevels >= 0 &&
             solve_4d_prim_core<WP>(uu00, lag, s0, pp0, dt, cx, WE_ENERGY, WF_LAB, ppw, &iters);
   if (ok) {
-    imp_commit(g, d, ph, cidx(g, ix, iy, iz), gm, ppw, iters);
+    s.solved(ph, t, gm, ppw, iters, 0, WP, WE_ENERGY, WF_LAB);
   } else {
-    unsigned long long k = atomicAdd(d.imp_nslow, 1ULL);
-    d.imp_slow[k] = (int32_t)t;
+    unsigned long long k = atomicAdd(w.nslow, 1ULL);
+    w.slow[k] = (int32_t)t;
   }
 }
-__global__ void __launch_bounds__(64) k_imp_slow(GridD g, Dev d, kz_physics ph, int sslot) {
+template <class S>
+__global__ void __launch_bounds__(64) k_imp_slow(S s, ImpWork w, kz_physics ph) {
   long i = TID;
-  if (i >= (long)*d.imp_nslow) return;
-  long nd = (long)g.nx * g.ny * g.nz;
-  long t = d.imp_slow[i];
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  long c = cidx(g, ix, iy, iz);
+  // the first-rung lists have been consumed; clear them for the next pass
+  if (i == 0) w.ncnt[0] = w.ncnt[1] = 0;
+  if (i >= (long)*w.nslow) return;
+  const long nd = s.count();
+  long t = w.slow[i];
   double uu00[NV], pp0[NV];
   State0 s0;
   LagOpac lag;
-  imp_load(d, nd, t, uu00, pp0, s0, lag);
-  const int sw = (int)d.imp_aux[IA_SW * nd + t];
+  imp_load(w, nd, t, uu00, pp0, s0, lag);
+  const int sw = (int)w.aux[IA_SW * nd + t];
   const int ot = (sw == WP_RAD) ? WP_MHD : WP_RAD;
   Geo gm;
-  geo_c(g, d, ix, iy, iz, gm);
-  double dt = d.ss[sslot];
+  s.geo(t, gm);
+  double dt = s.dt(t);
   const kz_implicit_params &ip = ph.implicit;
   ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ip, 1.0};
   for (int level = 0; level <= ip.opdamp_maxlevels; ++level) {
@@ -1013,13 +1096,12 @@ __global__ void __launch_bounds__(64) k_imp_slow(GridD g, Dev d, kz_physics ph, 
       bool ok = (wp == WP_MHD) ? solve_4d_prim_core<WP_MHD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters)
                                : solve_4d_prim_core<WP_RAD>(uu00, lag, s0, pp0, dt, cx, we, wf, ppw, &iters);
       if (ok) {
-        imp_commit(g, d, ph, c, gm, ppw, iters);
+        s.solved(ph, t, gm, ppw, iters, ir, wp, we, wf);
         return;
       }
     }
   }
-  d.flags[KZ_FLAG_RADIMP_FIXUP * g.ncell + c] = -1;
-  atomicAdd(&d.cnt[0], 1ULL);
+  s.failed(t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1780,6 +1862,8 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   e |= dalloc(s, (void **)&D.imp_aux, (size_t)nd * IMP_NAUX * sizeof(double));
   e |= dalloc(s, (void **)&D.imp_slow, (size_t)nd * sizeof(int32_t));
   e |= dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long));
+  e |= dalloc(s, (void **)&D.imp_list, (size_t)2 * nd * sizeof(int32_t));
+  e |= dalloc(s, (void **)&D.imp_ncnt, 2 * sizeof(unsigned long long));
   e |= dalloc(s, (void **)&D.sync, SY_N * sizeof(unsigned));
   e |= dalloc(s, (void **)&D.fix_list, (size_t)nd * sizeof(int32_t));
   if (ph->radviscosity || ph->dynamo) e |= dalloc(s, (void **)&D.bl, (size_t)G.ncell * BL_NC * sizeof(double));
@@ -1975,15 +2059,21 @@ static void launch_rijvisc(kz_sim *s, int dt_slot) {
   k_rijvisc<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor);
 }
 
+static ImpWork imp_work(const Dev &d) {
+  return ImpWork{d.imp_uu00, d.imp_pp0, d.imp_aux, d.imp_slow, d.imp_list, d.imp_nslow, d.imp_ncnt};
+}
+
 static void launch_op_implicit(kz_sim *s, int dt_slot) {
   if (!s->ph.has_opac) return;
   {
     TimedLaunch tl(s, KZ_K_IMPLICIT);
     unsigned nb = nblk(ndom(s), 64);
-    k_imp_prep<<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
-    k_imp_rung0<WP_MHD><<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
-    k_imp_rung0<WP_RAD><<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
-    k_imp_slow<<<nb, 64, 0, s->stream>>>(s->g, s->d, s->ph, dt_slot);
+    const GridCells cells{s->g, s->d, dt_slot};
+    const ImpWork w = imp_work(s->d);
+    k_imp_prep<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
+    k_imp_rung0<GridCells, WP_MHD><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
+    k_imp_rung0<GridCells, WP_RAD><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
+    k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
   }
   if (s->ph.do_fixups && s->ph.do_radimp_fixups)
     k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);
@@ -2192,6 +2282,158 @@ int kz_sim_kernel_time(kz_sim *s, double *ms_total, uint64_t *launches) {
   s->ev_used = 0;
   *ms_total = tot;
   *launches = n;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Independent-cell implicit batches (solveImplicitLab over n records, device
+// resident): the split, compacted solve of opImplicit over BatchCells.
+struct kz_implicit_batch {
+  long n = 0;
+  kz_physics ph;
+  hipStream_t stream = nullptr;
+  double *gp = nullptr, *dt = nullptr, *uu = nullptr, *pp = nullptr, *pp_in = nullptr, *stage = nullptr;
+  kz_implicit_result *res = nullptr;
+  unsigned long long *cnt = nullptr;
+  ImpWork w{};
+  std::vector<void *> allocs;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool has_input = false;
+};
+
+static int balloc(kz_implicit_batch *b, void **p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) return KZ_E_OOM;
+  b->allocs.push_back(*p);
+  return hipMemset(*p, 0, bytes) == hipSuccess ? 0 : KZ_E_HIP;
+}
+
+__global__ void k_batch_reset(long n, const double *pp_in, double *pp, double *uu, kz_implicit_result *res,
+                              unsigned long long *cnt) {
+  long t = TID;
+  if (t < 3) cnt[t] = 0;
+  if (t >= n) return;
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    pp[iv * n + t] = pp_in[iv * n + t];
+    uu[iv * n + t] = 0.0;
+  }
+  res[t] = kz_implicit_result{0, -1, 0, WP_RAD, WE_ENERGY, WF_LAB};
+}
+
+extern "C" {
+
+kz_implicit_batch *kz_implicit_batch_create(int64_t n, const kz_physics *ph, int *err) {
+  *err = 0;
+  if (n <= 0 || n > INT32_MAX) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
+  if (!ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
+  if (ph->opac.synchrotron_bridge) { *err = KZ_E_UNSUPPORTED; return nullptr; }
+  kz_implicit_batch *b = new kz_implicit_batch();
+  b->n = n;
+  b->ph = *ph;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    *err = KZ_E_HIP;
+    delete b;
+    return nullptr;
+  }
+  int e = 0;
+  e |= balloc(b, (void **)&b->gp, (size_t)n * (GEO_NC + 1) * sizeof(double));
+  e |= balloc(b, (void **)&b->dt, (size_t)n * sizeof(double));
+  for (double **f : {&b->uu, &b->pp, &b->pp_in, &b->stage}) e |= balloc(b, (void **)f, (size_t)n * NV * sizeof(double));
+  e |= balloc(b, (void **)&b->res, (size_t)n * sizeof(kz_implicit_result));
+  e |= balloc(b, (void **)&b->cnt, 4 * sizeof(unsigned long long));
+  e |= balloc(b, (void **)&b->w.uu00, (size_t)n * NV * sizeof(double));
+  e |= balloc(b, (void **)&b->w.pp0, (size_t)n * NV * sizeof(double));
+  e |= balloc(b, (void **)&b->w.aux, (size_t)n * IMP_NAUX * sizeof(double));
+  e |= balloc(b, (void **)&b->w.slow, (size_t)n * sizeof(int32_t));
+  e |= balloc(b, (void **)&b->w.list, (size_t)2 * n * sizeof(int32_t));
+  e |= balloc(b, (void **)&b->w.nslow, sizeof(unsigned long long));
+  e |= balloc(b, (void **)&b->w.ncnt, 2 * sizeof(unsigned long long));
+  for (auto &x : b->ev) e |= hipEventCreate(&x) != hipSuccess;
+  if (e) {
+    *err = e < 0 ? e : KZ_E_HIP;
+    kz_implicit_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void kz_implicit_batch_destroy(kz_implicit_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void *p : b->allocs) (void)hipFree(p);
+  for (auto x : b->ev)
+    if (x) (void)hipEventDestroy(x);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int kz_implicit_batch_set(kz_implicit_batch *b, const double *pp, const kz_geom *g, const double *dt) {
+  const long n = b->n;
+  std::vector<double> planes((size_t)n * (GEO_NC + 1));
+  for (long i = 0; i < n; ++i) {
+    const kz_geom &k = g[i];
+    for (int a = 0; a < 4; ++a)
+      for (int c = a; c < 4; ++c) {
+        if (bits_of(k.gg[a][c]) != bits_of(k.gg[c][a])) return KZ_E_INVALID_CONFIG;
+        planes[(size_t)tri(a, c) * n + i] = k.gg[a][c];
+      }
+    for (int a = 0; a < 4; ++a)
+      for (int c = 0; c < 4; ++c) planes[(size_t)(10 + a * 4 + c) * n + i] = k.GG[a][c];
+    planes[(size_t)26 * n + i] = k.gdet;
+    planes[(size_t)27 * n + i] = k.gttpert;
+    planes[(size_t)28 * n + i] = k.alpha;
+    planes[(size_t)GEO_NC * n + i] = k.xx[1];
+  }
+  HIPCHK(hipMemcpyAsync(b->gp, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->dt, dt, (size_t)n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->stage, pp, (size_t)n * NV * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  k_aos2soa<<<nblk(n * NV, 256), 256, 0, b->stream>>>(b->stage, b->pp_in, n, NV);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->has_input = true;
+  return 0;
+}
+
+int kz_implicit_batch_solve(kz_implicit_batch *b, double *ms) {
+  if (!b->has_input) return KZ_E_INVALID_CONFIG;
+  const long n = b->n;
+  k_batch_reset<<<nblk(n < 3 ? 3 : n, 256), 256, 0, b->stream>>>(n, b->pp_in, b->pp, b->uu, b->res, b->cnt);
+  const BatchCells cells{n, b->gp, b->dt, b->uu, b->pp, b->res, b->cnt};
+  const unsigned nb = nblk(n, 64);
+  HIPCHK(hipEventRecord(b->ev[0], b->stream));
+  k_imp_prep<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_rung0<BatchCells, WP_MHD><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_rung0<BatchCells, WP_RAD><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_slow<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  HIPCHK(hipEventRecord(b->ev[1], b->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (ms) {
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, b->ev[0], b->ev[1]));
+    *ms = f;
+  }
+  return 0;
+}
+
+int kz_implicit_batch_get(kz_implicit_batch *b, double *uu, double *pp, kz_implicit_result *r, uint64_t *counts) {
+  const long n = b->n;
+  const size_t fb = (size_t)n * NV * sizeof(double);
+  if (uu) {
+    k_soa2aos<<<nblk(n * NV, 256), 256, 0, b->stream>>>(b->uu, b->stage, n, NV);
+    HIPCHK(hipMemcpyAsync(uu, b->stage, fb, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  if (pp) {
+    k_soa2aos<<<nblk(n * NV, 256), 256, 0, b->stream>>>(b->pp, b->stage, n, NV);
+    HIPCHK(hipMemcpyAsync(pp, b->stage, fb, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  if (r) HIPCHK(hipMemcpyAsync(r, b->res, (size_t)n * sizeof(kz_implicit_result), hipMemcpyDeviceToHost, b->stream));
+  if (counts) HIPCHK(hipMemcpyAsync(counts, b->cnt, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

@@ -47,6 +47,12 @@ _SIGS = {
     "kz_calc_rij_visc_total": (C.c_int, [C.c_void_p, C.c_double]),
     "kz_sim_get_rijvisc": (C.c_int, [C.c_void_p, D]),
     "kz_apply_dynamo": (C.c_int, [C.c_void_p, C.c_double]),
+    # kz_implicit.h
+    "kz_implicit_batch_create": (C.c_void_p, [C.c_int64, C.POINTER(T.kz_physics), I32]),
+    "kz_implicit_batch_destroy": (None, [C.c_void_p]),
+    "kz_implicit_batch_set": (C.c_int, [C.c_void_p, D, GEOM, D]),
+    "kz_implicit_batch_solve": (C.c_int, [C.c_void_p, D]),
+    "kz_implicit_batch_get": (C.c_int, [C.c_void_p, D, D, C.c_void_p, C.POINTER(C.c_uint64)]),
     # kz_cellops.h
     "kz_p2u_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, D]),
     "kz_u2p_mhd_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.c_double, C.POINTER(T.kz_floor_params), C.c_void_p]),
@@ -217,6 +223,46 @@ class Sim:
         ms, n = C.c_double(0.0), C.c_uint64(0)
         _chk(lib().kz_sim_kernel_time(self.h, C.byref(ms), C.byref(n)), "kz_sim_kernel_time")
         return ms.value, n.value
+
+
+class ImplicitBatch:
+    """n independent implicit solves resident on the device (kz_implicit.h)."""
+
+    def __init__(self, n, phys):
+        self.n = int(n)
+        err = C.c_int32(0)
+        self.h = lib().kz_implicit_batch_create(self.n, C.byref(phys), C.byref(err))
+        if not self.h:
+            raise RuntimeError(f"kz_implicit_batch_create failed: {T.ERRORS.get(err.value, err.value)}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().kz_implicit_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set(self, pp, geom, dt):
+        pp, dt = _f64(pp), _f64(dt)
+        assert pp.shape == (self.n, T.NV) and dt.shape == (self.n,)
+        _chk(lib().kz_implicit_batch_set(self.h, T.ptr(pp), T.ptr(_geoms(geom)), T.ptr(dt)), "kz_implicit_batch_set")
+
+    def solve(self):
+        """Solve all cells from the loaded inputs; returns the kernel time in ms."""
+        ms = C.c_double(0.0)
+        _chk(lib().kz_implicit_batch_solve(self.h, C.byref(ms)), "kz_implicit_batch_solve")
+        return ms.value
+
+    def get(self):
+        """(uu, pp, result dict, counts (failures, iterations, solves))."""
+        uu = np.zeros((self.n, T.NV))
+        pp = np.zeros((self.n, T.NV))
+        r = np.zeros(self.n, T.IMPLICIT_RESULT_DTYPE)
+        cnt = (C.c_uint64 * 3)()
+        _chk(lib().kz_implicit_batch_get(self.h, T.ptr(uu), T.ptr(pp), r.ctypes.data_as(C.c_void_p), cnt),
+             "kz_implicit_batch_get")
+        return uu, pp, {k: r[k] for k in r.dtype.names}, tuple(int(c) for c in cnt)
 
 
 KERNELS = {"wavespeeds": 0, "face_flux": 1, "ct": 2, "update": 3, "u2p": 4, "implicit": 5, "bc": 6, "visc": 7,

@@ -32,6 +32,7 @@ def lib():
         L.kz_host_face_count.argtypes = [G, C.c_int]
         L.kz_host_metric_tables.argtypes = [G, C.c_double, C.c_double, C.c_double] + [D] * 9
         L.kz_host_metric_point.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D]
+        L.kz_host_geom_batch.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, D, D]
         L.kz_host_dxdx.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, D, D]
         L.kz_host_bc_xhi_tables.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
         L.kz_host_puffy_prepinit.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -71,6 +72,15 @@ def metric_point(coords, x4, mp=MP_PUFFY):
     out = np.zeros(101)
     x = np.ascontiguousarray(x4, np.float64)
     lib().kz_host_metric_point(coords, mp["a"], mp["mksr0"], mp["mksh0"], T.ptr(x), T.ptr(out))
+    return out
+
+
+def geom_batch(coords, x4, mp=MP_PUFFY):
+    """kz_geom records (GEOM_DOUBLES each) at the points x4[n, 4]."""
+    x = np.ascontiguousarray(x4, np.float64)
+    n = len(x)
+    out = np.zeros((n, T.GEOM_DOUBLES))
+    lib().kz_host_geom_batch(coords, mp["a"], mp["mksr0"], mp["mksh0"], n, T.ptr(x), T.ptr(out))
     return out
 
 

@@ -913,6 +913,25 @@ int kz_host_metric_point(int coords, double a, double mksr0, double mksh0, const
   return 0;
 }
 
+// kz_geom records (MetricCache.fillGeometry, precompute.zig:499-507) at n
+// arbitrary points x4[4 n]: the 20-double blocks of storeBlocks, gdet,
+// alpha = sqrt(-1/G^00), gttpert and the position.
+int kz_host_geom_batch(int coords, double a, double mksr0, double mksh0, int64_t n, const double *x4,
+                       kz_geom *out) {
+  MP mp{a, mksr0, mksh0};
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    CoordData d = compute(coords, mp, x4 + 4 * i);
+    kz_geom &o = out[i];
+    store_blocks(&o.gg[0][0], &o.GG[0][0], d);
+    o.gdet = d.gdet;
+    o.alpha = std::sqrt(-1.0 / d.gcon[0][0]);
+    o.gttpert = d.gttpert;
+    for (int k = 0; k < 4; ++k) o.xx[k] = x4[4 * i + k];
+  }
+  return 0;
+}
+
 // Jacobians for the metric goldens: dir 0 = MKS2->BL, 1 = BL->MKS2.
 int kz_host_dxdx(int dir, double a, double mksr0, double mksh0, const double *x4, double *out16) {
   MP mp{a, mksr0, mksh0};
