@@ -6,8 +6,9 @@
  * Storage mirrors the reference exactly: AoS fields iv-fastest over the
  * padded grid (koral/field.zig), FaceStore (+1 slice in its own dim,
  * sim/storage.zig:62-135), MetricCache blocks (metric/precompute.zig:121-526)
- * passed in by the caller.  Radiative viscosity and the mimic dynamo are not
- * part of this path (their Options must be off); 2D (nz == 1) only.
+ * passed in by the caller.  Also radiative viscosity (sim/rijvisc.zig,
+ * physics/radvisc.zig) and the mimic dynamo (sim/dynamo.zig); 2D and 3D
+ * grids (periodic z, fillCorners3d).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -105,7 +106,7 @@ kzo_sim *kzo_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g
                         const double *kris, const double *gb0, const double *gconb0, const double *gb1,
                         const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
-  if (gr->nz != 1 || gr->ny <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return 0; }
+  if (gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return 0; }
   if (ph->reduceorderatbh || ph->reduceorderafterfixup ||
       ph->opac.synchrotron_bridge || ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return 0; }
   if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return 0; } /* sim.zig:335 */
@@ -284,9 +285,15 @@ static int puffy_bc(kzo_sim *s, long ix, long iy, long iz, int face, double *pp)
   } else if (face == 2) { /* ylo */
     memcpy(pp, cell(s->p, s, ix, -iy - 1, iz), NV * sizeof(double));
     pp[VY] = -pp[VY]; pp[B2] = -pp[B2]; pp[FY] = -pp[FY];
-  } else { /* yhi */
+  } else if (face == 3) { /* yhi */
     memcpy(pp, cell(s->p, s, ix, ny - (iy - ny) - 1, iz), NV * sizeof(double));
     pp[VY] = -pp[VY]; pp[B2] = -pp[B2]; pp[FY] = -pp[FY];
+  } else { /* zlo / zhi: periodic (bc.zig:230-235, with its nz < ng pin) */
+    long nz = s->grid.nz, iiz = iz;
+    if (iz < 0) iiz = iz + nz;
+    if (iz > nz - 1) iiz = iz - nz;
+    if (nz < s->grid.ng) iiz = 0;
+    memcpy(pp, cell(s->p, s, ix, iy, iiz), NV * sizeof(double));
   }
   return KZO_OK;
 }
@@ -346,6 +353,57 @@ static void fill_corners_2d(kzo_sim *s) {
   avg_p(s, nx + 1, ny + 1, nx, ny + 1, nx + 1, ny); p2u_cell(s, nx + 1, ny + 1, 0);
 }
 
+/* bc.zig:378-520 fillCorners3d (serial: every z face is physical): the 12
+ * edge tubes (along z, then y, then x), each filled at every slice of its
+ * through-axis over the full ghost span, then the 8 corner cubes copied from
+ * the nearest domain corner, which overwrite the shared cells last. */
+static void copy3(kzo_sim *s, const long d[3], const long a[3]) {
+  memcpy(cell(s->p, s, d[0], d[1], d[2]), cell(s->p, s, a[0], a[1], a[2]), NV * sizeof(double));
+  p2u_cell(s, d[0], d[1], d[2]);
+}
+static void avg3(kzo_sim *s, const long d[3], const long a[3], const long b[3]) {
+  double *pa = cell(s->p, s, a[0], a[1], a[2]), *pb = cell(s->p, s, b[0], b[1], b[2]), pp[NV];
+  for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
+  memcpy(cell(s->p, s, d[0], d[1], d[2]), pp, sizeof pp);
+  p2u_cell(s, d[0], d[1], d[2]);
+}
+static void set3(long o[3], int ax, long av, int bx, long bv, int cx, long cv) { o[ax] = av; o[bx] = bv; o[cx] = cv; }
+static void edge_tube(kzo_sim *s, int pa, int a_hi, int pb, int b_hi, int pc) {
+  long ng = s->grid.ng;
+  long dims[3] = {s->grid.nx, s->grid.ny, s->grid.nz};
+  long na = dims[pa], nb = dims[pb], nc = dims[pc];
+  long a_edge = a_hi ? na - 1 : 0, a_g1 = a_hi ? na : -1, a_g2 = a_hi ? na + 1 : -2;
+  long b_edge = b_hi ? nb - 1 : 0, b_g1 = b_hi ? nb : -1, b_g2 = b_hi ? nb + 1 : -2;
+  for (long c = -ng; c < nc + ng; ++c) {
+    long d[3], x[3], y[3];
+    for (long i = 0; i < ng - 1; ++i) {
+      long a_out = a_hi ? na + i + 1 : -ng + i, b_out = b_hi ? nb + i + 1 : -ng + i;
+      set3(d, pa, a_out, pb, b_g1, pc, c); set3(x, pa, a_out, pb, b_edge, pc, c); copy3(s, d, x);
+      set3(d, pa, a_g1, pb, b_out, pc, c); set3(x, pa, a_edge, pb, b_out, pc, c); copy3(s, d, x);
+    }
+    set3(d, pa, a_g1, pb, b_g1, pc, c); set3(x, pa, a_g1, pb, b_edge, pc, c); set3(y, pa, a_edge, pb, b_g1, pc, c);
+    avg3(s, d, x, y);
+    set3(d, pa, a_g2, pb, b_g2, pc, c); set3(x, pa, a_g2, pb, b_g1, pc, c); set3(y, pa, a_g1, pb, b_g2, pc, c);
+    avg3(s, d, x, y);
+  }
+}
+static void corner_cube(kzo_sim *s, int x_hi, int y_hi, int z_hi) {
+  long ng = s->grid.ng, nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  long src[3] = {x_hi ? nx - 1 : 0, y_hi ? ny - 1 : 0, z_hi ? nz - 1 : 0};
+  for (long ix = x_hi ? nx : -ng; ix < (x_hi ? nx + ng : 0); ++ix)
+    for (long iy = y_hi ? ny : -ng; iy < (y_hi ? ny + ng : 0); ++iy)
+      for (long iz = z_hi ? nz : -ng; iz < (z_hi ? nz + ng : 0); ++iz) {
+        long d[3] = {ix, iy, iz};
+        copy3(s, d, src);
+      }
+}
+static void fill_corners_3d(kzo_sim *s) {
+  for (int k = 0; k < 4; ++k) edge_tube(s, 0, k >> 1, 1, k & 1, 2);
+  for (int k = 0; k < 4; ++k) edge_tube(s, 0, k >> 1, 2, k & 1, 1);
+  for (int k = 0; k < 4; ++k) edge_tube(s, 1, k >> 1, 2, k & 1, 0);
+  for (int k = 0; k < 8; ++k) corner_cube(s, k & 1, (k >> 1) & 1, (k >> 2) & 1);
+}
+
 /* bc.zig:40-73 setBc */
 int kzo_sim_set_bc(kzo_sim *s, int ifinit) {
   (void)ifinit;
@@ -366,14 +424,24 @@ int kzo_sim_set_bc(kzo_sim *s, int ifinit) {
         e = set_bc_cell(s, ix, ny - 1 + i, iz, 3);
         if (e) return e;
       }
-  fill_corners_2d(s);
+  for (long iy = 0; iy < ny; ++iy)
+    for (long ix = 0; ix < nx; ++ix)
+      for (long i = 1; i <= s->ngz; ++i) {
+        int e = set_bc_cell(s, ix, iy, -i, 4);
+        if (e) return e;
+        e = set_bc_cell(s, ix, iy, nz - 1 + i, 5);
+        if (e) return e;
+      }
+  if (nz == 1) fill_corners_2d(s);
+  else fill_corners_3d(s);
   return KZO_OK;
 }
 
 /* ---- p2u over the domain (Sim.initCell's p2u) ---- */
 void kzo_sim_p2u_domain(kzo_sim *s) {
-  for (long iy = 0; iy < s->grid.ny; ++iy)
-    for (long ix = 0; ix < s->grid.nx; ++ix) p2u_cell(s, ix, iy, 0);
+  for (long iz = 0; iz < s->grid.nz; ++iz)
+    for (long iy = 0; iy < s->grid.ny; ++iy)
+      for (long ix = 0; ix < s->grid.nx; ++ix) p2u_cell(s, ix, iy, iz);
 }
 
 /* ---- wavespeeds (sim.zig:747-886) ---- */
@@ -397,10 +465,9 @@ static void save_wavespeeds(kzo_sim *s, long ix, long iy, long iz, const double 
   double wsx = kzm_fmax(ax, kzm_fmax(kzm_fabs(rad0[0]), kzm_fabs(rad0[1])));
   double wsy = kzm_fmax(ay, kzm_fmax(kzm_fabs(rad0[2]), kzm_fabs(rad0[3])));
   double wsz = kzm_fmax(az, kzm_fmax(kzm_fabs(rad0[4]), kzm_fabs(rad0[5])));
-  (void)wsz;
   if (!(ix >= 0 && ix < s->grid.nx && iy >= 0 && iy < s->grid.ny && iz >= 0 && iz < s->grid.nz)) return;
-  double dx = csize(s, ix, 0), dy = csize(s, iy, 1);
-  double tsd = wsx / dx + wsy / dy; /* nz == 1, ny > 1 */
+  double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, iz, 2);
+  double tsd = s->grid.nz > 1 ? wsx / dx + wsy / dy + wsz / dz : wsx / dx + wsy / dy; /* ny > 1 */
   tsd /= s->ph.tsteplim;
   *scp(s, S_TSTEPDEN, ix, iy, iz) = tsd;
   if (tsd > *tmax) *tmax = tsd;
@@ -451,51 +518,54 @@ static int calc_wavespeeds(kzo_sim *s) {
 /* sim.zig:693-718 initTimestepGuess (2D) */
 void kzo_sim_init_timestep_guess(kzo_sim *s) {
   const double ws = 10000.0;
-  double tsd = ws / s->min_dx + ws / s->min_dy;
+  double tsd = s->grid.nz > 1 ? ws / s->min_dx + ws / s->min_dy + ws / s->min_dz : ws / s->min_dx + ws / s->min_dy;
   tsd /= s->ph.tsteplim;
   s->tstepdenmax = tsd;
   s->tstepdenmin = tsd;
-  for (long iy = 0; iy < s->grid.ny; ++iy)
-    for (long ix = 0; ix < s->grid.nx; ++ix) {
-      *scp(s, S_TSTEPDEN, ix, iy, 0) = tsd;
-      *scp(s, S_CELL_DT, ix, iy, 0) = 1.0 / tsd;
-    }
+  for (long iz = 0; iz < s->grid.nz; ++iz)
+    for (long iy = 0; iy < s->grid.ny; ++iy)
+      for (long ix = 0; ix < s->grid.nx; ++ix) {
+        *scp(s, S_TSTEPDEN, ix, iy, iz) = tsd;
+        *scp(s, S_CELL_DT, ix, iy, iz) = 1.0 / tsd;
+      }
 }
 
 static void save_timesteps(kzo_sim *s) {
-  for (long iy = 0; iy < s->grid.ny; ++iy)
-    for (long ix = 0; ix < s->grid.nx; ++ix)
-      *scp(s, S_CELL_DT, ix, iy, 0) = 1.0 / *scp(s, S_TSTEPDEN, ix, iy, 0);
+  for (long iz = 0; iz < s->grid.nz; ++iz)
+    for (long iy = 0; iy < s->grid.ny; ++iy)
+      for (long ix = 0; ix < s->grid.nx; ++ix)
+        *scp(s, S_CELL_DT, ix, iy, iz) = 1.0 / *scp(s, S_TSTEPDEN, ix, iy, iz);
 }
 
 /* ---- u2p pass + fixups (sim.zig:924-1133) ---- */
 static void u2p_rows(kzo_sim *s) {
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy) {
     int polar = polar_row(s, iy);
     for (long ix = 0; ix < s->grid.nx; ++ix) {
       double uu[NV], pp[NV];
-      memcpy(uu, cell(s->u, s, ix, iy, 0), sizeof uu);
-      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
-      *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0) = 0;
-      *flagp(s, KZ_FLAG_ENTROPY2, ix, iy, 0) = 0;
+      memcpy(uu, cell(s->u, s, ix, iy, iz), sizeof uu);
+      memcpy(pp, cell(s->p, s, ix, iy, iz), sizeof pp);
+      *flagp(s, KZ_FLAG_ENTROPY, ix, iy, iz) = 0;
+      *flagp(s, KZ_FLAG_ENTROPY2, ix, iy, iz) = 0;
       kzo_geo g;
-      geo_c(s, ix, iy, 0, &g);
+      geo_c(s, ix, iy, iz, &g);
       if (polar) {
         double gi = 1.0 / g.gdet;
         pp[B1] = uu[B1] * gi; pp[B2] = uu[B2] * gi; pp[B3] = uu[B3] * gi;
-        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
-        *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, 0) = 0;
-        *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, 0) = 0;
+        memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
+        *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, iz) = 0;
+        *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, iz) = 0;
         continue;
       }
       kz_u2p_result r = kzo_u2p_mhd(uu, pp, &g, s->ph.gam, &s->ph.floors);
-      if (r.entropy_used) *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0) = 1;
+      if (r.entropy_used) *flagp(s, KZ_FLAG_ENTROPY, ix, iy, iz) = 1;
       kz_u2prad_result rr = kzo_u2p_rad(uu, pp, &g, &s->ph.rad);
       kzo_check_floors_mhd(pp, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
       kzo_check_floors_rad(pp, &g, &s->ph.rad);
-      memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
-      *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, 0) = r.fixup ? 1 : 0;
-      *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, 0) = rr.corrected ? -1 : 0;
+      memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
+      *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, iz) = r.fixup ? 1 : 0;
+      *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, iz) = rr.corrected ? -1 : 0;
     }
   }
 }
@@ -507,20 +577,22 @@ static void cell_fixup(kzo_sim *s, int which) {
   if (!enabled) return;
   long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
   int any = 0;
-  for (long iy = 0; iy < ny && !any; ++iy)
-    for (long ix = 0; ix < nx; ++ix)
-      if (*flagp(s, which, ix, iy, 0) != 0) { any = 1; break; }
+  for (long iz = 0; iz < nz && !any; ++iz)
+    for (long iy = 0; iy < ny && !any; ++iy)
+      for (long ix = 0; ix < nx; ++ix)
+        if (*flagp(s, which, ix, iy, iz) != 0) { any = 1; break; }
   if (!any) return;
   size_t fb = (size_t)s->ncell * NV * sizeof(double);
   memcpy(s->u_bak, s->u, fb);
   memcpy(s->p_bak, s->p, fb);
+  for (long iz = 0; iz < nz; ++iz)
   for (long iy = 0; iy < ny; ++iy) {
     if (polar_row(s, iy)) continue;
     for (long ix = 0; ix < nx; ++ix) {
-      if (*flagp(s, which, ix, iy, 0) == 0) continue;
+      if (*flagp(s, which, ix, iy, iz) == 0) continue;
       double ppn[6][NV];
       int in_n = 0;
-      long nb[6][3] = {{ix - 1, iy, 0}, {ix + 1, iy, 0}, {ix, iy - 1, 0}, {ix, iy + 1, 0}, {ix, iy, -1}, {ix, iy, 1}};
+      long nb[6][3] = {{ix - 1, iy, iz}, {ix + 1, iy, iz}, {ix, iy - 1, iz}, {ix, iy + 1, iz}, {ix, iy, iz - 1}, {ix, iy, iz + 1}};
       for (int k = 0; k < 6; ++k) {
         if (nb[k][0] < 0 || nb[k][0] >= nx || nb[k][1] < 0 || nb[k][1] >= ny || nb[k][2] < 0 || nb[k][2] >= nz) continue;
         if (*flagp(s, which, nb[k][0], nb[k][1], nb[k][2]) != 0) continue;
@@ -530,7 +602,7 @@ static void cell_fixup(kzo_sim *s, int which) {
       int enough = (nz == 1 && ny == 1 && in_n >= 1) || (nz == 1 && in_n >= 2) || (ny == 1 && in_n >= 2) || in_n >= 3;
       if (!enough) continue;
       double pp[NV], uu[NV];
-      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+      memcpy(pp, cell(s->p, s, ix, iy, iz), sizeof pp);
       for (int iv = 0; iv < NV; ++iv) {
         int fixthis;
         if (which == KZ_FLAG_HD_FIXUP) fixthis = iv != RHO && iv < B1;
@@ -543,10 +615,10 @@ static void cell_fixup(kzo_sim *s, int which) {
         }
       }
       kzo_geo g;
-      geo_c(s, ix, iy, 0, &g);
+      geo_c(s, ix, iy, iz, &g);
       kzo_p2u(pp, uu, &g, s->ph.gam);
-      memcpy(cell(s->u_bak, s, ix, iy, 0), uu, sizeof uu);
-      memcpy(cell(s->p_bak, s, ix, iy, 0), pp, sizeof pp);
+      memcpy(cell(s->u_bak, s, ix, iy, iz), uu, sizeof uu);
+      memcpy(cell(s->p_bak, s, ix, iy, iz), pp, sizeof pp);
     }
   }
   memcpy(s->u, s->u_bak, fb);
@@ -913,49 +985,75 @@ static int mimic_dynamo(kzo_sim *s, double dt) {
         s->dyn_a[cidx(s, ix, iy, iz)] = aphi;
         if (dp->dampbeta) pp[B3] = bphi_new;
       }
-  /* ct.zig:228-264 cornerAverageA (2D) of (0, 0, Delta A_phi) */
-  for (long iy = 0; iy <= ny; ++iy)
-    for (long ix = 0; ix <= nx; ++ix) {
-      double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
-      v[0] = 1.0 / 4.0 * (0.0 + 0.0 + 0.0 + 0.0);
-      v[1] = v[0];
-      v[2] = 1.0 / 4.0 * (s->dyn_a[cidx(s, ix, iy, 0)] + s->dyn_a[cidx(s, ix, iy - 1, 0)] +
-                          s->dyn_a[cidx(s, ix - 1, iy, 0)] + s->dyn_a[cidx(s, ix - 1, iy - 1, 0)]);
-    }
-  /* ct.zig:279-331 calcBfromACore (2D) then dynamo.zig:432-460 superpose */
-  for (long iy = 0; iy < ny; ++iy)
-    for (long ix = 0; ix < nx; ++ix) {
-      double dA[4][4] = {{0}};
-      for (int c = 0; c < 3; ++c) {
-        const double a00 = s->vecpot[cidx(s, ix, iy, 0) * 6 + c], a01 = s->vecpot[cidx(s, ix, iy + 1, 0) * 6 + c];
-        const double a10 = s->vecpot[cidx(s, ix + 1, iy, 0) * 6 + c],
-                     a11 = s->vecpot[cidx(s, ix + 1, iy + 1, 0) * 6 + c];
-        dA[c + 1][2] = -(a00 - a01 + a10 - a11) / (2.0 * csize(s, iy, 1));
-        dA[c + 1][1] = -(a00 + a01 - a10 - a11) / (2.0 * csize(s, ix, 0));
-        dA[c + 1][3] = 0;
+  /* ct.zig:228-264 cornerAverageA of (0, 0, Delta A_phi) */
+  for (long iz = 0; iz <= nz; ++iz)
+    for (long iy = 0; iy <= ny; ++iy)
+      for (long ix = 0; ix <= nx; ++ix) {
+        if (nz == 1 && iz > 0) continue;
+        double *v = s->vecpot + cidx(s, ix, iy, iz) * 6;
+        for (int c = 0; c < 3; ++c) {
+#define DA(jx, jy, jz) (c == 2 ? s->dyn_a[cidx(s, jx, jy, jz)] : 0.0)
+          if (nz == 1)
+            v[c] = 1.0 / 4.0 * (DA(ix, iy, iz) + DA(ix, iy - 1, iz) + DA(ix - 1, iy, iz) + DA(ix - 1, iy - 1, iz));
+          else
+            v[c] = 1.0 / 8.0 * (DA(ix, iy, iz) + DA(ix, iy - 1, iz) + DA(ix - 1, iy, iz) + DA(ix - 1, iy - 1, iz) +
+                                DA(ix, iy, iz - 1) + DA(ix, iy - 1, iz - 1) + DA(ix - 1, iy, iz - 1) +
+                                DA(ix - 1, iy - 1, iz - 1));
+#undef DA
+        }
       }
-      double gdet = s->g[cidx(s, ix, iy, 0) * 20 + 19];
-      double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
-      v[3] = (dA[2][3] - dA[3][2]) / gdet;
-      v[4] = (dA[3][1] - dA[1][3]) / gdet;
-      v[5] = (dA[1][2] - dA[2][1]) / gdet;
-    }
-  for (long iy = 0; iy < ny; ++iy)
-    for (long ix = 0; ix < nx; ++ix) {
-      double *pp = s->p + cidx(s, ix, iy, 0) * NV;
-      const double *v = s->vecpot + cidx(s, ix, iy, 0) * 6;
-      pp[B1] += v[3];
-      pp[B2] += v[4];
-      pp[B3] += v[5];
-      kzo_geo geom;
-      geo_c(s, ix, iy, 0, &geom);
-      double uu[NV];
-      kzo_p2u(pp, uu, &geom, s->ph.gam);
-      double *u = s->u + cidx(s, ix, iy, 0) * NV;
-      u[B1] = uu[B1];
-      u[B2] = uu[B2];
-      u[B3] = uu[B3];
-    }
+  /* ct.zig:279-331 calcBfromACore then dynamo.zig:432-460 superpose */
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        double dA[4][4] = {{0}};
+        for (int c = 0; c < 3; ++c) {
+#define VA(jx, jy, jz) (s->vecpot[cidx(s, jx, jy, jz) * 6 + c])
+          if (nz == 1) {
+            dA[c + 1][2] = -(VA(ix, iy, iz) - VA(ix, iy + 1, iz) + VA(ix + 1, iy, iz) - VA(ix + 1, iy + 1, iz)) /
+                           (2.0 * csize(s, iy, 1));
+            dA[c + 1][1] = -(VA(ix, iy, iz) + VA(ix, iy + 1, iz) - VA(ix + 1, iy, iz) - VA(ix + 1, iy + 1, iz)) /
+                           (2.0 * csize(s, ix, 0));
+            dA[c + 1][3] = 0;
+          } else {
+            dA[c + 1][1] = (VA(ix + 1, iy, iz) - VA(ix, iy, iz) + VA(ix + 1, iy + 1, iz) - VA(ix, iy + 1, iz) +
+                            VA(ix + 1, iy, iz + 1) - VA(ix, iy, iz + 1) + VA(ix + 1, iy + 1, iz + 1) -
+                            VA(ix, iy + 1, iz + 1)) /
+                           (4.0 * csize(s, ix, 0));
+            dA[c + 1][2] = (VA(ix, iy + 1, iz) - VA(ix, iy, iz) + VA(ix + 1, iy + 1, iz) - VA(ix + 1, iy, iz) +
+                            VA(ix, iy + 1, iz + 1) - VA(ix, iy, iz + 1) + VA(ix + 1, iy + 1, iz + 1) -
+                            VA(ix + 1, iy, iz + 1)) /
+                           (4.0 * csize(s, iy, 1));
+            dA[c + 1][3] = (VA(ix, iy, iz + 1) - VA(ix, iy, iz) + VA(ix + 1, iy, iz + 1) - VA(ix + 1, iy, iz) +
+                            VA(ix, iy + 1, iz + 1) - VA(ix, iy + 1, iz) + VA(ix + 1, iy + 1, iz + 1) -
+                            VA(ix + 1, iy + 1, iz)) /
+                           (4.0 * csize(s, iz, 2));
+          }
+#undef VA
+        }
+        double gdet = s->g[cidx(s, ix, iy, iz) * 20 + 19];
+        double *v = s->vecpot + cidx(s, ix, iy, iz) * 6;
+        v[3] = (dA[2][3] - dA[3][2]) / gdet;
+        v[4] = (dA[3][1] - dA[1][3]) / gdet;
+        v[5] = (dA[1][2] - dA[2][1]) / gdet;
+      }
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        double *pp = s->p + cidx(s, ix, iy, iz) * NV;
+        const double *v = s->vecpot + cidx(s, ix, iy, iz) * 6;
+        pp[B1] += v[3];
+        pp[B2] += v[4];
+        pp[B3] += v[5];
+        kzo_geo geom;
+        geo_c(s, ix, iy, iz, &geom);
+        double uu[NV];
+        kzo_p2u(pp, uu, &geom, s->ph.gam);
+        double *u = s->u + cidx(s, ix, iy, iz) * NV;
+        u[B1] = uu[B1];
+        u[B2] = uu[B2];
+        u[B3] = uu[B3];
+      }
   return KZO_OK;
 }
 
@@ -1133,17 +1231,18 @@ void kzo_flux_ct(kzo_sim *s) {
 }
 
 static void update_rows(kzo_sim *s, double dtin) {
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
       double ms[NV];
-      metric_source(s, ix, iy, 0, ms);
-      double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, 0, 2);
+      metric_source(s, ix, iy, iz, ms);
+      double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, iz, 2);
       double dt = dtin;
-      double *flxl = face(s->flb[0], s, 0, ix, iy, 0), *flxr = face(s->flb[0], s, 0, ix + 1, iy, 0);
-      double *flyl = face(s->flb[1], s, 1, ix, iy, 0), *flyr = face(s->flb[1], s, 1, ix, iy + 1, 0);
+      double *flxl = face(s->flb[0], s, 0, ix, iy, iz), *flxr = face(s->flb[0], s, 0, ix + 1, iy, iz);
+      double *flyl = face(s->flb[1], s, 1, ix, iy, iz), *flyr = face(s->flb[1], s, 1, ix, iy + 1, iz);
       /* z faces of a collapsed dim: the store is zeroed by fluxesAtFaces */
-      double *flzl = face(s->flb[2], s, 2, ix, iy, 0), *flzr = face(s->flb[2], s, 2, ix, iy, 1);
-      double *uu = cell(s->u, s, ix, iy, 0);
+      double *flzl = face(s->flb[2], s, 2, ix, iy, iz), *flzr = face(s->flb[2], s, 2, ix, iy, iz + 1);
+      double *uu = cell(s->u, s, ix, iy, iz);
       for (int iv = 0; iv < NV; ++iv) {
         double du = -(flxr[iv] - flxl[iv]) * dt / dx - (flyr[iv] - flyl[iv]) * dt / dy - (flzr[iv] - flzl[iv]) * dt / dz;
         uu[iv] = uu[iv] + du + ms[iv] * dt;
@@ -1167,23 +1266,24 @@ int kzo_sim_op_explicit(kzo_sim *s, double dt) {
 /* ---- implicit operator (sim.zig:1715-1776) ---- */
 int kzo_sim_op_implicit(kzo_sim *s, double dtin) {
   if (!s->ph.has_opac) return KZO_OK;
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy) {
     if (polar_row(s, iy)) continue;
     for (long ix = 0; ix < s->grid.nx; ++ix) {
       double uu[NV], pp[NV];
-      memcpy(uu, cell(s->u, s, ix, iy, 0), sizeof uu);
-      memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+      memcpy(uu, cell(s->u, s, ix, iy, iz), sizeof uu);
+      memcpy(pp, cell(s->p, s, ix, iy, iz), sizeof pp);
       kzo_geo g;
-      geo_c(s, ix, iy, 0, &g);
+      geo_c(s, ix, iy, iz, &g);
       kz_implicit_result r = kzo_solve_implicit_lab(uu, pp, &g, dtin, s->ph.gam, &s->ph.rad, &s->ph.opac, &s->ph.implicit);
       if (r.ok) {
-        memcpy(cell(s->u, s, ix, iy, 0), uu, sizeof uu);
-        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
-        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, 0) = 0;
+        memcpy(cell(s->u, s, ix, iy, iz), uu, sizeof uu);
+        memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
+        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, iz) = 0;
         s->n_iters += (uint64_t)r.iters;
         s->n_solves += 1;
       } else {
-        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, 0) = -1;
+        *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, iz) = -1;
         s->n_fail += 1;
       }
     }
@@ -1196,6 +1296,7 @@ int kzo_sim_op_implicit(kzo_sim *s, double dtin) {
 int kzo_sim_do_correct(kzo_sim *s) {
   if (!s->ph.correct_polaraxis) return KZO_OK;
   long nc = s->ph.nccorrectpolar, ny = s->grid.ny;
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long ix = 0; ix < s->grid.nx; ++ix)
     for (long ic = 0; ic < nc; ++ic) {
       long rows2[2] = {ic, ny - 1 - ic};
@@ -1205,16 +1306,16 @@ int kzo_sim_do_correct(kzo_sim *s) {
         double thaxis = yl_(s, iy < nc ? 0 : ny);
         double fac = kzm_fabs((yc_(s, iy) - thaxis) / (yc_(s, iysrc) - thaxis));
         double pp[NV], uu[NV];
-        double *ps = cell(s->p, s, ix, iysrc, 0);
-        memcpy(pp, cell(s->p, s, ix, iy, 0), sizeof pp);
+        double *ps = cell(s->p, s, ix, iysrc, iz);
+        memcpy(pp, cell(s->p, s, ix, iy, iz), sizeof pp);
         pp[RHO] = ps[RHO]; pp[UU] = ps[UU]; pp[ENTR] = ps[ENTR];
         pp[VX] = ps[VX]; pp[VZ] = ps[VZ]; pp[VY] = fac * ps[VY];
         pp[EE] = ps[EE]; pp[FX] = ps[FX]; pp[FZ] = ps[FZ]; pp[FY] = fac * ps[FY];
         kzo_geo g;
-        geo_c(s, ix, iy, 0, &g);
+        geo_c(s, ix, iy, iz, &g);
         kzo_p2u(pp, uu, &g, s->ph.gam);
-        memcpy(cell(s->p, s, ix, iy, 0), pp, sizeof pp);
-        memcpy(cell(s->u, s, ix, iy, 0), uu, sizeof uu);
+        memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
+        memcpy(cell(s->u, s, ix, iy, iz), uu, sizeof uu);
       }
     }
   return KZO_OK;
@@ -1223,35 +1324,39 @@ int kzo_sim_do_correct(kzo_sim *s) {
 /* ---- stage arithmetic (sim.zig:1586-1709) ---- */
 static void stage_deriv(kzo_sim *s, double *dst, const double *a, const double *b, double delta) {
   double f1 = 1.0 / delta, f2 = -1.0 / delta;
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
-      long o = cidx(s, ix, iy, 0) * NV;
+      long o = cidx(s, ix, iy, iz) * NV;
       for (int iv = 0; iv < NV; ++iv) dst[o + iv] = f1 * a[o + iv] + f2 * b[o + iv];
     }
 }
 static void stage_combine(kzo_sim *s, double *dst, const double *a, double f1, const double *b, double f2, const double *c) {
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
-      long o = cidx(s, ix, iy, 0) * NV;
+      long o = cidx(s, ix, iy, iz) * NV;
       for (int iv = 0; iv < NV; ++iv) dst[o + iv] = a[o + iv] + f1 * b[o + iv] + f2 * c[o + iv];
     }
 }
 static void copy_full(kzo_sim *s, double *dst, const double *src) { memcpy(dst, src, (size_t)s->ncell * NV * sizeof(double)); }
 
 static void copy_entropy_count(kzo_sim *s) {
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix)
-      *flagp(s, KZ_FLAG_ENTROPY3, ix, iy, 0) = *flagp(s, KZ_FLAG_ENTROPY, ix, iy, 0);
+      *flagp(s, KZ_FLAG_ENTROPY3, ix, iy, iz) = *flagp(s, KZ_FLAG_ENTROPY, ix, iy, iz);
 }
 
 static void update_entropy(kzo_sim *s) {
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
-      double *pp = cell(s->p, s, ix, iy, 0);
+      double *pp = cell(s->p, s, ix, iy, iz);
       pp[ENTR] = kzo_s_from_u(pp[RHO], pp[UU], s->ph.gam);
       kzo_geo g;
-      geo_c(s, ix, iy, 0, &g);
-      kzo_p2u_mhd(pp, cell(s->u, s, ix, iy, 0), &g, s->ph.gam);
+      geo_c(s, ix, iy, iz, &g);
+      kzo_p2u_mhd(pp, cell(s->u, s, ix, iy, iz), &g, s->ph.gam);
     }
 }
 
@@ -1317,11 +1422,12 @@ int kzo_sim_step(kzo_sim *s, const double *forced_dt) {
  * pmag/ptot) over the domain and refresh u. Ghosts keep the unscaled B. */
 double kzo_sim_postinit(kzo_sim *s, double maxbeta) {
   double maxb = 0.0;
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
-      const double *pp = cell(s->p, s, ix, iy, 0);
+      const double *pp = cell(s->p, s, ix, iy, iz);
       kzo_geo g;
-      geo_c(s, ix, iy, 0, &g);
+      geo_c(s, ix, iy, iz, &g);
       double ucon[4], ucov[4];
       kzo_ucon_ucov_from_prims(pp, &g, ucon, ucov);
       double B[3] = {pp[B1], pp[B2], pp[B3]}, bcon[4], bcov[4], bsq;
@@ -1335,11 +1441,12 @@ double kzo_sim_postinit(kzo_sim *s, double maxbeta) {
       if (pmag / ptot > maxb) maxb = pmag / ptot;
     }
   double fac = kzm_sqrt(maxbeta / maxb);
+  for (long iz = 0; iz < s->grid.nz; ++iz)
   for (long iy = 0; iy < s->grid.ny; ++iy)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
-      double *pp = cell(s->p, s, ix, iy, 0);
+      double *pp = cell(s->p, s, ix, iy, iz);
       pp[B1] *= fac; pp[B2] *= fac; pp[B3] *= fac;
-      p2u_cell(s, ix, iy, 0);
+      p2u_cell(s, ix, iy, iz);
     }
   return fac;
 }

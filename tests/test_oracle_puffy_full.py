@@ -4,15 +4,16 @@ Two reference anchors for the full production option set
 (koral/testing/selfscenarios.zig:71-90: polar-axis correction, radiative
 viscosity, dynamo, PUFFY BCs):
 
-* the C forced-dt step golden step/puffy64.kstp (64x60), driven the way the
-  reference's own test drives it (puffystep_golden_tests.zig:166-300): C's
+* the C forced-dt step goldens step/puffy64.kstp (64x60) and
+  step/puffy3d.kstp (32x30x4, periodic phi), driven the way the
+  reference's own tests drive them (puffystep_golden_tests.zig:149-300): C's
   post-init state loaded bit-for-bit, ghosts rebuilt, C's dt sequence forced;
   gated by that test's own lines — flag maps exact, >= 95% of cells within
   1e-3 field-scale, no cell above 0.15, the first step within 1e-4;
-* the Zig self-golden selfgolden/puffy_fast.kslf (32x30, 3 CFL steps from
-  the Zig init).  The oracle's init reproduces Zig's to ~1e-10 (libm), so the
-  gate is the flag map, the CFL clock and a field-scale bound rather than
-  bits.
+* the Zig self-goldens selfgolden/puffy_{fast,full}.kslf (32x30, 3 CFL
+  steps; 48x44, 8 CFL steps), stepped from Zig's own post-init snapshot;
+  gated by the flag map, the CFL clock and field-scale bounds set by the
+  oracle's measured sensitivity rather than bits (Zig's libm is not ours).
 """
 import numpy as np
 import pytest
@@ -33,14 +34,17 @@ def _full_phys():
 
 def _r_bl_col(grid):
     """Boyer-Lindquist radius of each domain column (MKS2: r = exp(x1) + r0)."""
-    bl = host.bl_tables(grid).reshape(grid.ny + 6, grid.nx + 6, host.BL_REC)
-    return bl[3, 3:3 + grid.nx, 34]
+    (sz, sy, sx), _ = T.padded_shape(grid)
+    bl = host.bl_tables(grid).reshape(sz, sy, sx, host.BL_REC)
+    return bl[0, 3, 3:3 + grid.nx, 34]
 
 
-@pytest.mark.slow
-def test_puffy64_forced_dt_vs_c():
-    k = read_kstp("step/puffy64.kstp.gz")
-    grid = T.grid_puffy(k["nx"], k["ny"])
+@pytest.mark.parametrize("name", ["puffy64.kstp.gz", "puffy3d.kstp.gz"])
+def test_forced_dt_vs_c(name):
+    """puffy64: 64x60; puffy3d: 32x30x4 (the pi/2 wedge with periodic phi,
+    the 3D corner fill, the z sweep and the 3D curl of the dynamo)."""
+    k = read_kstp("step/" + name)
+    grid = T.grid_puffy(k["nx"], k["ny"], k["nz"])
     phys = _full_phys()
     tab = host.metric_tables(grid)
     sim = kzo.Sim(grid, phys, tab)
@@ -48,7 +52,8 @@ def test_puffy64_forced_dt_vs_c():
     p, u, _ = sim.get_state()
     r0 = k["recs"][0]
     ng = 3
-    dom = (slice(None), slice(ng, ng + grid.ny), slice(ng, ng + grid.nx))
+    ngz = ng if grid.nz > 1 else 0
+    dom = (slice(ngz, ngz + grid.nz), slice(ng, ng + grid.ny), slice(ng, ng + grid.nx))
     p[dom] = r0["p"]
     u[dom] = r0["u"]
     sim.set_state(p=p, u=u)
@@ -79,7 +84,7 @@ def test_puffy64_forced_dt_vs_c():
         interior = (rbl[None, None, :] >= 2.5) & (iy > nc) & (iy < grid.ny - 1 - nc)
         report.append((i, float(dev.max()), float(np.where(interior, dev, 0).max()), float(over),
                        float(flags_ok.mean())))
-    print("puffy64 step, max dev, interior, cells>1e-3, flags:", report)
+    print(name, "step, max dev, interior, cells>1e-3, flags:", report)
     for i, mx, inner, over, fl in report:
         assert fl == 1.0, report
         assert over <= 0.05, report
