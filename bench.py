@@ -142,6 +142,35 @@ def bench_implicit_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def face_units(grid):
+    """Faces one k_face_flux launch computes (every active dimension)."""
+    n = (grid.nx, grid.ny, grid.nz)
+    tot = 0
+    for d in range(3):
+        if n[d] <= 1:
+            continue
+        a, b = (n[1], n[2]) if d == 0 else ((n[0], n[2]) if d == 1 else (n[0], n[1]))
+        tot += (n[d] + 1) * (a + 2 if a > 1 else a) * (b + 2 if b > 1 else b)
+    return tot
+
+
+def cpu_baseline_fresh(grid, phys, mp, seed, rho_u_only, steps):
+    """The oracle (single thread) over `steps` CFL steps of `grid` from its
+    own seeded init: zone-cycles/s on this host."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import kzo
+    from koral_zig_amd import host
+    tab = host.metric_tables(grid, mp)
+    sim = kzo.Sim(grid, phys, tab)
+    host.init_puffy(sim, grid, tab, phys, seed=seed, rho_u_only=rho_u_only)
+    sim.step()  # the startup step (tiny dt) is not timed
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    return grid.nx * grid.ny * grid.nz * steps / dt, dt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,8 +181,12 @@ def main():
     ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--latency-steps", type=int, default=50)
-    ap.add_argument("--workload", choices=["puffy", "implicit-batch"], default="puffy",
-                    help="puffy: the 384x360 step (default); implicit-batch: the independent-cell solve batch")
+    ap.add_argument("--workload", choices=["puffy", "wedge3d", "implicit-batch"], default="puffy",
+                    help="puffy: the 384x360 step (default); wedge3d: the 3D wedge; implicit-batch: the "
+                         "independent-cell solve batch")
+    ap.add_argument("--nx3", type=int, default=192)
+    ap.add_argument("--ny3", type=int, default=192)
+    ap.add_argument("--nz3", type=int, default=64)
     ap.add_argument("--cells", type=int, default=1 << 22, help="implicit-batch: number of cells")
     ap.add_argument("--cpu-cells", type=int, default=20000, help="implicit-batch: oracle sample size (0: skip)")
     args = ap.parse_args()
@@ -170,15 +203,49 @@ def main():
             args.warmup = 1
         bench_implicit_batch(args)
         return
+    if args.workload == "wedge3d":
+        if "--steps" not in sys.argv:
+            args.steps = 200
+        if "--warmup" not in sys.argv:
+            args.warmup = 5
+        if "--latency-steps" not in sys.argv:
+            args.latency_steps = 10
     from koral_zig_amd import abi_types as T
-    from koral_zig_amd import hip, host
+    from koral_zig_amd import host
 
+    if args.workload == "wedge3d":
+        # the 3D wedge: 192x192x64 (+3 ghosts), periodic phi over pi/2, spin
+        # 0.9375 (RMIN from rminForSpin), rho and u perturbed
+        spin = 0.9375
+        nx, ny, nz = args.nx3, args.ny3, args.nz3
+        grid = T.grid_puffy(nx, ny, nz, rmin=T.rmin_for_spin(spin))
+        phys = T.physics_puffy(bh_spin=spin)
+        mp = dict(host.MP_PUFFY, a=spin)
+        label = (f"PUFFY 3D wedge {nx}x{ny}x{nz} MKS2 a={spin} M=10Msun, periodic phi over pi/2, seeded 1e-4 "
+                 f"perturbation of rho and u, CFL steps, radviscosity/dynamo off")
+        metric = f"zone-cycles/s full RMHD+M1 step, 3D wedge {nx}x{ny}x{nz} on 1 GPU"
+        cpu_grid = T.grid_puffy(48, 48, 16, rmin=T.rmin_for_spin(spin))
+        run_grid(args, grid, phys, mp, metric, label, True, cpu_grid)
+        return
     grid = T.grid_puffy(args.nx, args.ny)
     phys = T.physics_puffy()
+    label = (f"PUFFY torus {args.nx}x{args.ny}x1 MKS2 a=0 M=10Msun, seeded 1e-4 perturbation, "
+             f"CFL steps, radviscosity/dynamo off")
+    metric = "zone-cycles/s full RMHD+M1 step at 384x360 on 1 GPU; p50/p99 step latency batched"
+    run_grid(args, grid, phys, host.MP_PUFFY, metric, label, False, None)
+
+
+def run_grid(args, grid, phys, mp, metric, label, rho_u_only, cpu_grid):
+    """Time K CFL steps of a PUFFY run on the GPU and print the JSON line;
+    cpu_grid (optional) is a smaller grid of the same physics for the oracle
+    sample when the run itself is too large to step on one host thread."""
+    from koral_zig_amd import hip, host
+
     t0 = time.perf_counter()
-    tab = host.metric_tables(grid)
+    tab = host.metric_tables(grid, mp)
+    log(f"metric tables {time.perf_counter() - t0:.1f}s")
     sim = hip.Sim(grid, phys, tab)
-    host.init_puffy(sim, grid, tab, phys, seed=args.seed)
+    host.init_puffy(sim, grid, tab, phys, seed=args.seed, rho_u_only=rho_u_only)
     log(f"setup {time.perf_counter() - t0:.1f}s (metric tables, PUFFY init, perturbation seed {args.seed:#x})")
 
     sim.step_many(args.warmup)
@@ -207,15 +274,14 @@ def main():
     ms_dom, n_dom = sim.kernel_time()
     sim.time_kernel(None)
     c1 = sim.counters()
-    zones = grid.nx * grid.ny
+    zones = grid.nx * grid.ny * grid.nz
     value = zones * args.steps / wall
     log(f"timed {args.steps} steps in {wall:.3f}s: {value:.4e} zone-cycles/s; t={c1['t']:.6e}")
 
     avg_ms = ms_dom / max(n_dom, 1)
     if dom in BYTES_PER_UNIT:
         if dom == "face_flux":
-            units = (grid.nx + 1) * (grid.ny + 2) + (grid.ny + 1) * (grid.nx + 2)  # both dims, per pair
-            units_per_launch = units / 2.0
+            units_per_launch = face_units(grid)
         else:
             units_per_launch = zones
         achieved = BYTES_PER_UNIT[dom] * units_per_launch / (avg_ms * 1e-3) / 1e9
@@ -228,7 +294,8 @@ def main():
         achieved = flops / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = (TRAFFIC_BYTES_PER_LAUNCH_C1.get(dom) if (args.nx, args.ny) == (384, 360) else None)
+    c1 = (grid.nx, grid.ny, grid.nz) == (384, 360, 1)
+    roof["traffic"] = TRAFFIC_BYTES_PER_LAUNCH_C1.get(dom) if c1 else None
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_ms
     roof["launches"] = n_dom
@@ -243,17 +310,23 @@ def main():
     lat = np.array(lat)
 
     cpu = None
-    if args.cpu_steps > 0:
+    if args.cpu_steps > 0 and cpu_grid is None:
         p, u, _ = sim.get_state()
         v, secs = cpu_baseline(grid, phys, tab, p, u, sim.counters(), args.cpu_steps)
         cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} CFL steps of the same {args.nx}x{args.ny} run from the GPU's "
-                         f"post-warmup state ({secs:.1f}s, one thread)"}
+               "sample": f"{args.cpu_steps} CFL steps of the same {grid.nx}x{grid.ny}x{grid.nz} run from the "
+                         f"GPU's post-warmup state ({secs:.1f}s, one thread)"}
+        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+    elif args.cpu_steps > 0:
+        v, secs = cpu_baseline_fresh(cpu_grid, phys, mp, args.seed, rho_u_only, args.cpu_steps)
+        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} CFL steps of a {cpu_grid.nx}x{cpu_grid.ny}x{cpu_grid.nz} run of the "
+                         f"same physics from its own seeded init ({secs:.1f}s, one thread)"}
         log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
     sim.close()
 
     out = {
-        "metric": "zone-cycles/s full RMHD+M1 step at 384x360 on 1 GPU; p50/p99 step latency batched",
+        "metric": metric,
         "value": value,
         "unit": "zone-cycles/s",
         "n_gpus": 1,
@@ -265,9 +338,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"PUFFY torus {args.nx}x{args.ny}x1 MKS2 a=0 M=10Msun, seeded 1e-4 perturbation, "
-                               f"CFL steps, radviscosity/dynamo off",
-                   "nx": args.nx, "ny": args.ny, "seed": args.seed},
+        "config": {"workload": label, "nx": grid.nx, "ny": grid.ny, "nz": grid.nz, "seed": args.seed},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_latency_ms": ({"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),

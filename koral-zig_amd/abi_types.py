@@ -193,10 +193,10 @@ def dynamo_params_puffy():
                             calchronthego=1)
 
 
-def physics_puffy(mass=10.0, radviscosity=False, dynamo=False):
+def physics_puffy(mass=10.0, radviscosity=False, dynamo=False, bh_spin=0.0):
     """The per-cell knobs of Physics.toOptions (koral/problems/puffy/puffy.zig:210-242).
     Radiative viscosity and the mimic dynamo (on in PUFFY production runs) are
-    off unless asked for."""
+    off unless asked for; bh_spin sets r_H and r_ISCO for them."""
     return kz_physics(gam=5.0 / 3.0, tsteplim=0.5, minmod_theta=1.5,
                       floors=floor_params_puffy(), rad=rad_params_puffy(),
                       opac=opac_params_puffy(mass), implicit=implicit_params_puffy(),
@@ -204,7 +204,12 @@ def physics_puffy(mass=10.0, radviscosity=False, dynamo=False):
                       correct_polaraxis=1, nccorrectpolar=2, reduceorderatbh=0,
                       reduceorderafterfixup=0, dampradwavespeednearaxis=0,
                       radviscosity=int(radviscosity), dynamo=int(dynamo),
-                      radvisc=radvisc_params_puffy(), dynamo_params=dynamo_params_puffy(), bh_spin=0.0)
+                      radvisc=radvisc_params_puffy(), dynamo_params=dynamo_params_puffy(), bh_spin=bh_spin)
+
+
+def rmin_for_spin(a):
+    """puffy.zig:75-77 rminForSpin: RMIN(a) = 1.85 r_H(a) / r_H(0)."""
+    return 1.85 * (1.0 + math.sqrt(1.0 - a * a)) / 2.0
 
 
 def grid_puffy(nx, ny, nz=1, rmin=1.85, rmax=500.0, mksr0=0.1):

@@ -383,16 +383,160 @@ __device__ void bc_corners2d(const GridD &g, const Dev &d, const kz_physics &ph)
   }
 }
 
+// z faces, periodic in phi (puffy.zig:237, bc.zig:187-252 .periodic, with its
+// nz < ng pin); t < nx*ny*ngz*2
+__device__ void bc_z_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
+  const int side = t % 2;
+  const long i = (t / 2) % g.ngz + 1;
+  const long ix = (t / (2 * g.ngz)) % g.nx;
+  const long iy = t / (2L * g.ngz * g.nx);
+  const long nz = g.nz;
+  const long iz = side ? nz - 1 + i : -i;
+  long iiz = iz;
+  if (iz < 0) iiz = iz + nz;
+  if (iz > nz - 1) iiz = iz - nz;
+  if (nz < g.ng) iiz = 0;
+  double pp[NV], uu[NV];
+  ld(d.p, g.ncell, cidx(g, ix, iy, iiz), pp);
+  Geo geom;
+  geo_c(g, d, ix, iy, iz, geom);
+  p2u(pp, uu, geom, ph.gam);
+  const long c = cidx(g, ix, iy, iz);
+  st(d.p, g.ncell, c, pp);
+  st(d.u, g.ncell, c, uu);
+}
+// bc.zig:378-520 fillCorners3d.  The reference fills the 12 edge tubes in
+// turn (along z, y, x) over their full through-axis span and then the 8
+// corner cubes, which overwrite every tube cell that lies in a z (or y, x)
+// ghost slice of another tube; those cells are the only ones whose tube
+// value reads a not-yet-refreshed ghost.  The surviving tube cells are the
+// ones at domain slices of the through-axis, which read only face ghosts and
+// their own tube's copies, so the device fills those in two phases (copies
+// and inner diagonals, then outer diagonals) and the cubes alongside the
+// second phase; the result equals the reference's sequence.
+__device__ __forceinline__ void c3set(long o[3], int ax, long av, int bx, long bv, int cx, long cv) {
+  o[ax] = av;
+  o[bx] = bv;
+  o[cx] = cv;
+}
+__device__ void cp_p2u3(const GridD &g, const Dev &d, double gam, const long dst[3], const long src[3]) {
+  double pp[NV], uu[NV];
+  ld(d.p, g.ncell, cidx(g, src[0], src[1], src[2]), pp);
+  Geo gm;
+  geo_c(g, d, dst[0], dst[1], dst[2], gm);
+  p2u(pp, uu, gm, gam);
+  const long c = cidx(g, dst[0], dst[1], dst[2]);
+  st(d.p, g.ncell, c, pp);
+  st(d.u, g.ncell, c, uu);
+}
+__device__ void avg_p2u3(const GridD &g, const Dev &d, double gam, const long dst[3], const long a[3],
+                         const long b[3]) {
+  double pa[NV], pb[NV], pp[NV], uu[NV];
+  ld(d.p, g.ncell, cidx(g, a[0], a[1], a[2]), pa);
+  ld(d.p, g.ncell, cidx(g, b[0], b[1], b[2]), pb);
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
+  Geo gm;
+  geo_c(g, d, dst[0], dst[1], dst[2], gm);
+  p2u(pp, uu, gm, gam);
+  const long c = cidx(g, dst[0], dst[1], dst[2]);
+  st(d.p, g.ncell, c, pp);
+  st(d.u, g.ncell, c, uu);
+}
+// One (tube, corner, domain slice) of the 12 edge tubes: type 0 = x-y corners
+// along z, 1 = x-z along y, 2 = y-z along x; q = (a_hi, b_hi).
+struct EdgeSlice {
+  int pa, pb, pc;
+  long na, nb, c;
+  bool a_hi, b_hi;
+};
+__device__ __forceinline__ bool edge_slice(const GridD &g, long s, EdgeSlice &e) {
+  const long dims[3] = {g.nx, g.ny, g.nz};
+  const long n0 = 4L * g.nz, n1 = 4L * g.ny, n2 = 4L * g.nx;
+  int type;
+  if (s < n0) type = 0;
+  else if (s < n0 + n1) { type = 1; s -= n0; }
+  else if (s < n0 + n1 + n2) { type = 2; s -= n0 + n1; }
+  else return false;
+  const int q = (int)(s % 4);
+  e.c = s / 4;
+  e.pa = type == 2 ? 1 : 0;
+  e.pb = type == 0 ? 1 : 2;
+  e.pc = type == 0 ? 2 : (type == 1 ? 1 : 0);
+  e.na = dims[e.pa];
+  e.nb = dims[e.pb];
+  e.a_hi = q >> 1;
+  e.b_hi = q & 1;
+  return true;
+}
+__global__ void __launch_bounds__(64) k_bc_edges1(GridD g, Dev d, kz_physics ph) {
+  const long t = TID;
+  const long ng = g.ng;
+  const int per = 2 * (int)(ng - 1) + 1;
+  EdgeSlice e;
+  if (!edge_slice(g, t / per, e)) return;
+  const int k = (int)(t % per);
+  const long a_edge = e.a_hi ? e.na - 1 : 0, a_g1 = e.a_hi ? e.na : -1;
+  const long b_edge = e.b_hi ? e.nb - 1 : 0, b_g1 = e.b_hi ? e.nb : -1;
+  long dst[3], x[3], y[3];
+  if (k < 2 * (ng - 1)) {
+    const long i = k / 2;
+    if (k % 2 == 0) {  // (a_out, b_g1) <- (a_out, b_edge)
+      const long a_out = e.a_hi ? e.na + i + 1 : -ng + i;
+      c3set(dst, e.pa, a_out, e.pb, b_g1, e.pc, e.c);
+      c3set(x, e.pa, a_out, e.pb, b_edge, e.pc, e.c);
+    } else {  // (a_g1, b_out) <- (a_edge, b_out)
+      const long b_out = e.b_hi ? e.nb + i + 1 : -ng + i;
+      c3set(dst, e.pa, a_g1, e.pb, b_out, e.pc, e.c);
+      c3set(x, e.pa, a_edge, e.pb, b_out, e.pc, e.c);
+    }
+    cp_p2u3(g, d, ph.gam, dst, x);
+  } else {  // inner diagonal from the two face ghosts
+    c3set(dst, e.pa, a_g1, e.pb, b_g1, e.pc, e.c);
+    c3set(x, e.pa, a_g1, e.pb, b_edge, e.pc, e.c);
+    c3set(y, e.pa, a_edge, e.pb, b_g1, e.pc, e.c);
+    avg_p2u3(g, d, ph.gam, dst, x, y);
+  }
+}
+__global__ void __launch_bounds__(64) k_bc_edges2(GridD g, Dev d, kz_physics ph) {
+  const long t = TID;
+  const long ng = g.ng;
+  const long nslice = 4L * (g.nx + g.ny + g.nz);
+  if (t < nslice) {  // outer diagonal from two of the slice's copies
+    EdgeSlice e;
+    edge_slice(g, t, e);
+    const long a_g1 = e.a_hi ? e.na : -1, a_g2 = e.a_hi ? e.na + 1 : -2;
+    const long b_g1 = e.b_hi ? e.nb : -1, b_g2 = e.b_hi ? e.nb + 1 : -2;
+    long dst[3], x[3], y[3];
+    c3set(dst, e.pa, a_g2, e.pb, b_g2, e.pc, e.c);
+    c3set(x, e.pa, a_g2, e.pb, b_g1, e.pc, e.c);
+    c3set(y, e.pa, a_g1, e.pb, b_g2, e.pc, e.c);
+    avg_p2u3(g, d, ph.gam, dst, x, y);
+    return;
+  }
+  const long u = t - nslice, ng3 = ng * ng * ng;
+  if (u >= 8 * ng3) return;
+  const int cube = (int)(u / ng3);  // bit 0 x_hi, bit 1 y_hi, bit 2 z_hi
+  const long r = u % ng3;
+  const bool x_hi = cube & 1, y_hi = (cube >> 1) & 1, z_hi = (cube >> 2) & 1;
+  const long dst[3] = {(x_hi ? g.nx : -ng) + r % ng, (y_hi ? g.ny : -ng) + (r / ng) % ng,
+                       (z_hi ? g.nz : -ng) + r / (ng * ng)};
+  const long src[3] = {x_hi ? g.nx - 1 : 0, y_hi ? g.ny - 1 : 0, z_hi ? g.nz - 1 : 0};
+  cp_p2u3(g, d, ph.gam, dst, src);
+}
+
 // setBc (bc.zig:40-73): x ghosts (reading only domain cells of rows
-// 0..ny-1) and y ghosts (reading only domain cells) are independent, so one
-// launch does both; the corner fill, which reads both, follows as
-// k_bc_corners.
+// 0..ny-1), y ghosts and z ghosts (reading only domain cells) are
+// independent, so one launch does all three; the corner fill, which reads
+// them, follows (k_bc_corners in 2D, k_bc_edges1/2 in 3D).
 __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_BC) k_bc(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   long nxg = (long)g.nz * g.ny * g.ngx * 2;
   long nyg = (long)g.nz * g.nx * g.ngy * 2;
+  long nzg = (long)g.ny * g.nx * g.ngz * 2;
   if (t < nxg) bc_x_cell(g, d, ph, t);
   else if (t < nxg + nyg) bc_y_cell(g, d, ph, t - nxg);
+  else if (t < nxg + nyg + nzg) bc_z_cell(g, d, ph, t - nxg - nyg);
 }
 __global__ void __launch_bounds__(64) k_bc_corners(GridD g, Dev d, kz_physics ph) { bc_corners2d(g, d, ph); }
 
@@ -1461,24 +1605,38 @@ __global__ void __launch_bounds__(64) k_dynamo_a(GridD g, Dev d, kz_physics ph, 
   d.dyn_a[c] = aphi;
   if (ph.dynamo_params.dampbeta) d.p[B3 * n + c] = bphi_new;
 }
-// mimicDynamo, second half (2D): the corner average of (0, 0, Delta A_phi)
+// mimicDynamo, second half: the corner average of (0, 0, Delta A_phi)
 // (ct.zig:228-264 cornerAverageA), its curl (ct.zig:279-331 calcBfromACore)
 // and the superposition on the domain's B (dynamo.zig:432-460), with u[B]
-// from p2u.  The corner values are formed per cell from the four cells
-// around each corner, the reference's sum order.
-__device__ __forceinline__ double corner_a(const GridD &g, const Dev &d, long ix, long iy) {
-  return 1.0 / 4.0 *
-         (d.dyn_a[cidx(g, ix, iy, 0)] + d.dyn_a[cidx(g, ix, iy - 1, 0)] + d.dyn_a[cidx(g, ix - 1, iy, 0)] +
-          d.dyn_a[cidx(g, ix - 1, iy - 1, 0)]);
+// from p2u.  The corner values are formed per cell from the four (2D) or
+// eight (3D) cells around each corner, in the reference's sum order.
+__device__ __forceinline__ double dyn_a_at(const GridD &g, const Dev &d, long ix, long iy, long iz) {
+  return d.dyn_a[cidx(g, ix, iy, iz)];
+}
+// corner A_phi (cornerAverageA): 4 cells around the corner in 2D, 8 in 3D
+__device__ __forceinline__ double corner_a(const GridD &g, const Dev &d, long ix, long iy, long iz) {
+  if (g.nz == 1)
+    return 1.0 / 4.0 *
+           (dyn_a_at(g, d, ix, iy, iz) + dyn_a_at(g, d, ix, iy - 1, iz) + dyn_a_at(g, d, ix - 1, iy, iz) +
+            dyn_a_at(g, d, ix - 1, iy - 1, iz));
+  return 1.0 / 8.0 *
+         (dyn_a_at(g, d, ix, iy, iz) + dyn_a_at(g, d, ix, iy - 1, iz) + dyn_a_at(g, d, ix - 1, iy, iz) +
+          dyn_a_at(g, d, ix - 1, iy - 1, iz) + dyn_a_at(g, d, ix, iy, iz - 1) + dyn_a_at(g, d, ix, iy - 1, iz - 1) +
+          dyn_a_at(g, d, ix - 1, iy, iz - 1) + dyn_a_at(g, d, ix - 1, iy - 1, iz - 1));
 }
 __global__ void __launch_bounds__(128) k_dynamo_curl(GridD g, Dev d, kz_physics ph) {
   long t = TID;
-  if (t >= (long)g.nx * g.ny) return;
-  const long ix = t % g.nx, iy = t / g.nx;
-  const long n = g.ncell, c = cidx(g, ix, iy, 0);
-  const double z = 1.0 / 4.0 * (0.0 + 0.0 + 0.0 + 0.0);
-  const double acorner[4] = {corner_a(g, d, ix, iy), corner_a(g, d, ix, iy + 1), corner_a(g, d, ix + 1, iy),
-                             corner_a(g, d, ix + 1, iy + 1)};
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  const long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  const long n = g.ncell, c = cidx(g, ix, iy, iz);
+  const bool d3 = g.nz > 1;
+  // the A_r and A_theta corners are averages of zeros
+  const double z = d3 ? 1.0 / 8.0 * (0.0 + 0.0 + 0.0 + 0.0 + 0.0 + 0.0 + 0.0 + 0.0)
+                      : 1.0 / 4.0 * (0.0 + 0.0 + 0.0 + 0.0);
+  // corners (jx, jy, jz) in {ix, ix+1} x {iy, iy+1} x {iz, iz+1}: index jx-ix + 2 (jy-iy) + 4 (jz-iz)
+  double ac[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) ac[q] = (q >= 4 && !d3) ? 0.0 : corner_a(g, d, ix + (q & 1), iy + ((q >> 1) & 1), iz + (q >> 2));
   double dA[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -1486,11 +1644,18 @@ __global__ void __launch_bounds__(128) k_dynamo_curl(GridD g, Dev d, kz_physics 
     for (int j = 0; j < 4; ++j) dA[i][j] = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double a00 = k == 2 ? acorner[0] : z, a01 = k == 2 ? acorner[1] : z, a10 = k == 2 ? acorner[2] : z,
-                 a11 = k == 2 ? acorner[3] : z;
-    dA[k + 1][2] = -(a00 - a01 + a10 - a11) / (2.0 * csize(g, iy, 1));
-    dA[k + 1][1] = -(a00 + a01 - a10 - a11) / (2.0 * csize(g, ix, 0));
-    dA[k + 1][3] = 0;
+#define A_(q) (k == 2 ? ac[q] : z)
+    if (!d3) {
+      // A(ix,iy)=q0, A(ix,iy+1)=q2, A(ix+1,iy)=q1, A(ix+1,iy+1)=q3
+      dA[k + 1][2] = -(A_(0) - A_(2) + A_(1) - A_(3)) / (2.0 * csize(g, iy, 1));
+      dA[k + 1][1] = -(A_(0) + A_(2) - A_(1) - A_(3)) / (2.0 * csize(g, ix, 0));
+      dA[k + 1][3] = 0;
+    } else {
+      dA[k + 1][1] = (A_(1) - A_(0) + A_(3) - A_(2) + A_(5) - A_(4) + A_(7) - A_(6)) / (4.0 * csize(g, ix, 0));
+      dA[k + 1][2] = (A_(2) - A_(0) + A_(3) - A_(1) + A_(6) - A_(4) + A_(7) - A_(5)) / (4.0 * csize(g, iy, 1));
+      dA[k + 1][3] = (A_(4) - A_(0) + A_(5) - A_(1) + A_(6) - A_(2) + A_(7) - A_(3)) / (4.0 * csize(g, iz, 2));
+    }
+#undef A_
   }
   const double gdet = d.gc[26 * n + c];
   const double v3 = (dA[2][3] - dA[3][2]) / gdet;
@@ -1502,7 +1667,7 @@ __global__ void __launch_bounds__(128) k_dynamo_curl(GridD g, Dev d, kz_physics 
   pp[B2] += v4;
   pp[B3] += v5;
   Geo geom;
-  geo_c(g, d, ix, iy, 0, geom);
+  geo_c(g, d, ix, iy, iz, geom);
   p2u(pp, uu, geom, ph.gam);
   d.p[B1 * n + c] = pp[B1];
   d.p[B2 * n + c] = pp[B2];
@@ -1802,7 +1967,7 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
                       const double *kris, const double *gb0, const double *gconb0, const double *gb1,
                       const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
-  if (gr->nz != 1 || gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return nullptr; }
+  if (gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return nullptr; }
   if (ph->reduceorderatbh || ph->reduceorderafterfixup || ph->opac.synchrotron_bridge ||
       ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return nullptr; }
   if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
@@ -1979,12 +2144,23 @@ int kz_sim_p2u_domain(kz_sim *s) {
   return check_err(s);
 }
 
+// cells of the domain plus one ring in every active dimension
+static long ring_count(const GridD &g) {
+  return (long)(g.nx > 1 ? g.nx + 2 : g.nx) * (g.ny > 1 ? g.ny + 2 : g.ny) * (g.nz > 1 ? g.nz + 2 : g.nz);
+}
+
 static void launch_set_bc(kz_sim *s) {
   const GridD &g = s->g;
   TimedLaunch tl(s, KZ_K_BC);
-  long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2;
+  long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2 + (long)g.ny * g.nx * g.ngz * 2;
   k_bc<<<nblk(nb, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
-  if (g.nz == 1) k_bc_corners<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
+  if (g.nz == 1) {
+    k_bc_corners<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
+  } else {
+    const long nslice = 4L * (g.nx + g.ny + g.nz), per = 2L * (g.ng - 1) + 1;
+    k_bc_edges1<<<nblk(nslice * per, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
+    k_bc_edges2<<<nblk(nslice + 8L * g.ng * g.ng * g.ng, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
+  }
 }
 
 int kz_sim_set_bc(kz_sim *s, int ifinit) {
@@ -2010,13 +2186,19 @@ static void launch_calc_u2p(kz_sim *s, bool with_bc) {
 
 static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
   const GridD &g = s->g;
-  long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
+  long nw = ring_count(g);
   {
     TimedLaunch tl(s, KZ_K_WAVESPEEDS);
     k_wavespeeds<<<nblk(nw, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
   {
-    long nf = (long)(g.nx + 1) * (g.ny + 2) * g.nz + (long)(g.ny + 1) * (g.nx + 2) * g.nz;
+    long nf = 0;  // face_count over the three dimensions
+    const long nd3[3] = {g.nx, g.ny, g.nz};
+    for (int dim = 0; dim < 3; ++dim) {
+      if (nd3[dim] <= 1) continue;
+      const long n0 = dim == 0 ? g.ny : g.nx, n1 = dim == 2 ? g.ny : g.nz;
+      nf += (nd3[dim] + 1) * (n0 > 1 ? n0 + 2 : n0) * (n1 > 1 ? n1 + 2 : n1);
+    }
     TimedLaunch tl(s, KZ_K_FACE_FLUX);
     if (s->ph.radviscosity) k_face_flux<true><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
     else k_face_flux<false><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
@@ -2044,9 +2226,9 @@ static void launch_dynamo(kz_sim *s, int dt_slot, bool with_bc) {
   launch_set_bc(s);
   {
     TimedLaunch tl(s, KZ_K_DYNAMO);
-    long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
+    long nw = ring_count(g);
     k_dynamo_a<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor, s->risco);
-    k_dynamo_curl<<<nblk((long)g.nx * g.ny, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
+    k_dynamo_curl<<<nblk((long)g.nx * g.ny * g.nz, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
   launch_calc_u2p(s, with_bc);
 }
@@ -2055,7 +2237,7 @@ static void launch_dynamo(kz_sim *s, int dt_slot, bool with_bc) {
 static void launch_rijvisc(kz_sim *s, int dt_slot) {
   const GridD &g = s->g;
   TimedLaunch tl(s, KZ_K_VISC);
-  long nw = (long)(g.nx + 2) * (g.ny + 2) * g.nz;
+  long nw = ring_count(g);
   k_rijvisc<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor);
 }
 

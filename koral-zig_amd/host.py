@@ -133,12 +133,13 @@ def geometry_bl(grid, ix, iy, iz=0, mp=MP_PUFFY):
     return out
 
 
-def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-4):
+def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-4, rho_u_only=False):
     """The ko.c init sequence (puffy.zig:873-884 initAllWith) driven through
     an engine (the HIP sim handle, or the CPU restatement in tests):
     prepinit+init -> set_bc -> calc_BfromA -> set_bc -> postinit (beta norm)
     -> initTimestepGuess; with `seed`, the seeded 1e-4 perturbation of every
-    non-B primitive is applied after postinit, followed by p2u + set_bc.
+    non-B primitive (of rho and u only with rho_u_only) is applied after
+    postinit, followed by p2u + set_bc.
     Returns the beta-normalisation factor."""
     p = puffy_prepinit(grid, tables, phys, init)
     engine.set_state(p=p)
@@ -152,7 +153,7 @@ def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-
     fac = engine.postinit(init["maxbeta"])
     if seed is not None:
         p, _, _ = engine.get_state()
-        p = perturb(grid, np.ascontiguousarray(p), seed, amp, phys.gam)
+        p = perturb(grid, np.ascontiguousarray(p), seed, amp, phys.gam, rho_u_only)
         engine.set_state(p=p)
         engine.p2u_domain()
         engine.set_bc(False)

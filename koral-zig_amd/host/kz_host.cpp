@@ -991,12 +991,16 @@ int kz_host_puffy_prepinit(const kz_grid *gr, double a, double mksr0, double mks
   double uintatmmin = (op->kb_over_mugas_mp * rhoatmmin * atm_tgas) / (gam - 1.0);
   double t3 = atm_trad_init;
   double eradatmmin = op->four_sigmarad * t3 * t3 * t3 * t3 / 10.0 * 6.62 / mass;
-  // cells are independent (each runs its own torus quadrature): rows in parallel
-  const long nrows = (long)gr->nz * gr->ny;
+  // cells are independent (each runs its own torus quadrature): rows in
+  // parallel.  The torus, the atmosphere and the metric do not depend on phi,
+  // so in 3D the phi = first slice is computed and copied to the others
+  // (identical bits: every input of a cell but its phi coordinate, which no
+  // expression here reads, is the same in every slice).
+  const long nrows = gr->ny;
 #pragma omp parallel for schedule(dynamic, 1)
   for (long row = 0; row < nrows; ++row)
     for (long ix = 0; ix < gr->nx; ++ix) {
-      const long iz = row / gr->ny, iy = row % gr->ny;
+      const long iz = 0, iy = row;
       {
         long ci = G.cidx(ix, iy, iz);
         kz_geom geom;
@@ -1070,40 +1074,70 @@ int kz_host_puffy_prepinit(const kz_grid *gr, double a, double mksr0, double mks
         memcpy(p + ci * KZ_NV, pp, sizeof pp);
       }
     }
+  for (long iz = 1; iz < gr->nz; ++iz)
+    for (long iy = 0; iy < gr->ny; ++iy)
+      memcpy(p + G.cidx(0, iy, iz) * KZ_NV, p + G.cidx(0, iy, 0) * KZ_NV, (size_t)gr->nx * KZ_NV * sizeof(double));
   return 0;
 }
 
-// ct.zig:184-334 calcBfromA with overwrite (2D, nz == 1): cell-centred A in
-// the B slots of the padded p -> corner average -> curl -> B in every cell's
-// B slots (ghost cells get 0, as the reference's zeroed scratch gives them).
+// ct.zig:184-334 calcBfromA with overwrite: cell-centred A in the B slots of
+// the padded p -> corner average -> curl -> B in every cell's B slots (ghost
+// cells get 0, as the reference's zeroed scratch gives them).
 int kz_host_calc_b_from_a(const kz_grid *gr, const double *g20, double *p) {
   Grid G(*gr);
-  long nx = gr->nx, ny = gr->ny;
-  if (gr->nz != 1) return KZ_E_UNSUPPORTED;
+  const long nx = gr->nx, ny = gr->ny, nz = gr->nz;
+  if (ny <= 1) return KZ_E_UNSUPPORTED;
   std::vector<double> vp((size_t)G.ncell() * 6, 0.0);
-  auto A = [&](int c, long ix, long iy) { return p[G.cidx(ix, iy, 0) * KZ_NV + KZ_B1 + c]; };
-  for (long iy = 0; iy <= ny; ++iy)
-    for (long ix = 0; ix <= nx; ++ix)
-      for (int c = 0; c < 3; ++c)
-        vp[G.cidx(ix, iy, 0) * 6 + c] =
-            1.0 / 4.0 * (A(c, ix, iy) + A(c, ix, iy - 1) + A(c, ix - 1, iy) + A(c, ix - 1, iy - 1));
-  auto S = [&](int c, long ix, long iy) { return vp[G.cidx(ix, iy, 0) * 6 + c]; };
-  for (long iy = 0; iy < ny; ++iy)
-    for (long ix = 0; ix < nx; ++ix) {
-      double dA[4][4] = {{0}};
-      for (int c = 0; c < 3; ++c) {
-        dA[c + 1][2] = -(S(c, ix, iy) - S(c, ix, iy + 1) + S(c, ix + 1, iy) - S(c, ix + 1, iy + 1)) /
-                       (2.0 * G.csize(iy, 1));
-        dA[c + 1][1] = -(S(c, ix, iy) + S(c, ix, iy + 1) - S(c, ix + 1, iy) - S(c, ix + 1, iy + 1)) /
-                       (2.0 * G.csize(ix, 0));
-        dA[c + 1][3] = 0;
+  auto A = [&](int c, long ix, long iy, long iz) { return p[G.cidx(ix, iy, iz) * KZ_NV + KZ_B1 + c]; };
+  // ct.zig:228-264 cornerAverageA
+  for (long iz = 0; iz <= nz; ++iz)
+    for (long iy = 0; iy <= ny; ++iy)
+      for (long ix = 0; ix <= nx; ++ix) {
+        if (nz == 1 && iz > 0) continue;
+        for (int c = 0; c < 3; ++c)
+          vp[G.cidx(ix, iy, iz) * 6 + c] =
+              nz == 1 ? 1.0 / 4.0 * (A(c, ix, iy, iz) + A(c, ix, iy - 1, iz) + A(c, ix - 1, iy, iz) +
+                                     A(c, ix - 1, iy - 1, iz))
+                      : 1.0 / 8.0 * (A(c, ix, iy, iz) + A(c, ix, iy - 1, iz) + A(c, ix - 1, iy, iz) +
+                                     A(c, ix - 1, iy - 1, iz) + A(c, ix, iy, iz - 1) + A(c, ix, iy - 1, iz - 1) +
+                                     A(c, ix - 1, iy, iz - 1) + A(c, ix - 1, iy - 1, iz - 1));
       }
-      double gdet = g20[G.cidx(ix, iy, 0) * 20 + 19];
-      long ci = G.cidx(ix, iy, 0);
-      vp[ci * 6 + 3] = (dA[2][3] - dA[3][2]) / gdet;
-      vp[ci * 6 + 4] = (dA[3][1] - dA[1][3]) / gdet;
-      vp[ci * 6 + 5] = (dA[1][2] - dA[2][1]) / gdet;
-    }
+  // ct.zig:279-331 calcBfromACore
+  auto S = [&](int c, long ix, long iy, long iz) { return vp[G.cidx(ix, iy, iz) * 6 + c]; };
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        double dA[4][4] = {{0}};
+        for (int c = 0; c < 3; ++c) {
+          if (nz == 1) {
+            dA[c + 1][2] = -(S(c, ix, iy, iz) - S(c, ix, iy + 1, iz) + S(c, ix + 1, iy, iz) -
+                             S(c, ix + 1, iy + 1, iz)) /
+                           (2.0 * G.csize(iy, 1));
+            dA[c + 1][1] = -(S(c, ix, iy, iz) + S(c, ix, iy + 1, iz) - S(c, ix + 1, iy, iz) -
+                             S(c, ix + 1, iy + 1, iz)) /
+                           (2.0 * G.csize(ix, 0));
+            dA[c + 1][3] = 0;
+          } else {
+            dA[c + 1][1] = (S(c, ix + 1, iy, iz) - S(c, ix, iy, iz) + S(c, ix + 1, iy + 1, iz) -
+                            S(c, ix, iy + 1, iz) + S(c, ix + 1, iy, iz + 1) - S(c, ix, iy, iz + 1) +
+                            S(c, ix + 1, iy + 1, iz + 1) - S(c, ix, iy + 1, iz + 1)) /
+                           (4.0 * G.csize(ix, 0));
+            dA[c + 1][2] = (S(c, ix, iy + 1, iz) - S(c, ix, iy, iz) + S(c, ix + 1, iy + 1, iz) -
+                            S(c, ix + 1, iy, iz) + S(c, ix, iy + 1, iz + 1) - S(c, ix, iy, iz + 1) +
+                            S(c, ix + 1, iy + 1, iz + 1) - S(c, ix + 1, iy, iz + 1)) /
+                           (4.0 * G.csize(iy, 1));
+            dA[c + 1][3] = (S(c, ix, iy, iz + 1) - S(c, ix, iy, iz) + S(c, ix + 1, iy, iz + 1) -
+                            S(c, ix + 1, iy, iz) + S(c, ix, iy + 1, iz + 1) - S(c, ix, iy + 1, iz) +
+                            S(c, ix + 1, iy + 1, iz + 1) - S(c, ix + 1, iy + 1, iz)) /
+                           (4.0 * G.csize(iz, 2));
+          }
+        }
+        const long ci = G.cidx(ix, iy, iz);
+        const double gdet = g20[ci * 20 + 19];
+        vp[ci * 6 + 3] = (dA[2][3] - dA[3][2]) / gdet;
+        vp[ci * 6 + 4] = (dA[3][1] - dA[1][3]) / gdet;
+        vp[ci * 6 + 5] = (dA[1][2] - dA[2][1]) / gdet;
+      }
   for (long c = 0; c < G.ncell(); ++c) {
     p[c * KZ_NV + KZ_B1] = vp[c * 6 + 3];
     p[c * KZ_NV + KZ_B2] = vp[c * 6 + 4];

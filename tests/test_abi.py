@@ -64,13 +64,6 @@ def test_create_rejects_radvisc_without_opacities():
         hip.Sim(grid, phys, tab)
 
 
-def test_create_rejects_3d():
-    grid = T.grid_puffy(16, 12, 4)
-    tab = host.metric_tables(grid)
-    with pytest.raises(RuntimeError, match="Unsupported|-7"):
-        hip.Sim(grid, T.physics_puffy(), tab)
-
-
 def test_metric_tables_symmetric_bitwise():
     """The device keeps g and the Christoffels as upper triangles; MetricCache
     (and the host library restating it) builds them symmetric bit for bit."""
