@@ -97,6 +97,23 @@ void *kz_sim_stream(kz_sim *s);
  * tstepdenmax/min, nstep, implicit-solve counters). Synchronous. */
 int kz_sim_counters(kz_sim *s, kz_counters *c);
 
+/* KDMP v2 checkpoints (koral/io/dump.zig:19-160): kz_sim_write_kdmp
+ * serializes the header (domain shape, nv, the run's t and nstep, out_idx)
+ * and the domain primitives p[nz][ny][nx][nv] (serializePrimDump :101-135)
+ * into buf (kz_sim_kdmp_size bytes; KZ_E_TRUNCATED if cap is smaller);
+ * kz_sim_load_kdmp validates magic, version, shape and length
+ * (KZ_E_BAD_MAGIC / _BAD_VERSION / _DIM_MISMATCH / _TRUNCATED, the
+ * reference's ReadError), stores the primitives and u = p2u(p) in every
+ * domain cell (loadPrimDump :137-160) and restores t and nstep (the driver's
+ * restart, problems/puffy/main.zig:283-298); kz_sim_finish_init then fills
+ * the ghosts and the startup dt guess (Sim.finishInit, koral/sim.zig:
+ * 686-690).  kz_kdmp_parse_header is parseDumpHeader (:82-98). */
+int kz_kdmp_parse_header(const uint8_t *buf, uint64_t len, kz_dump_header *h);
+int kz_sim_kdmp_size(kz_sim *s, uint64_t *bytes);
+int kz_sim_write_kdmp(kz_sim *s, uint32_t out_idx, uint8_t *buf, uint64_t cap, uint64_t *written);
+int kz_sim_load_kdmp(kz_sim *s, const uint8_t *buf, uint64_t len, kz_dump_header *h);
+int kz_sim_finish_init(kz_sim *s);
+
 /* Kernel classes of a step, for event timing. */
 enum {
   KZ_K_WAVESPEEDS = 0, /* calcWavespeeds */

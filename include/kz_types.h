@@ -37,7 +37,11 @@ enum {
   KZ_E_OOM = -4,
   KZ_E_INVALID_CONFIG = -5,  /* Sim.init preconditions (koral/sim.zig:319-363) */
   KZ_E_HIP = -6,             /* a HIP runtime call failed */
-  KZ_E_UNSUPPORTED = -7      /* a reference option this build does not implement */
+  KZ_E_UNSUPPORTED = -7,     /* a reference option this build does not implement */
+  KZ_E_BAD_MAGIC = -8,       /* dump.ReadError.BadMagic (koral/io/dump.zig:43) */
+  KZ_E_BAD_VERSION = -9,     /* dump.ReadError.BadVersion */
+  KZ_E_DIM_MISMATCH = -10,   /* dump.ReadError.DimMismatch */
+  KZ_E_TRUNCATED = -11       /* dump.ReadError.Truncated */
 };
 
 /* Point geometry (koral/geometry.zig:29-49). gg/GG keep the reference's 4x5
@@ -207,5 +211,18 @@ typedef struct kz_counters {
   uint64_t n_radimp_failures, n_radimp_iters, n_radimp_solves;
   uint64_t n_guard_recoveries;
 } kz_counters;
+
+/* KDMP v2 checkpoint header (koral/io/dump.zig:29-45 DumpHeader): the
+ * domain cell counts, the primitive count, the clock, the step count and the
+ * output frame index.  On disk: "KDMP", u32 version 2, u32 nx ny nz nv, f64 t,
+ * u64 nstep, u32 out_idx, little-endian (44 bytes), then p[nz][ny][nx][nv]. */
+typedef struct kz_dump_header {
+  uint32_t nx, ny, nz, nv;
+  double t;
+  uint64_t nstep;
+  uint32_t out_idx;
+  uint32_t _pad;
+} kz_dump_header;
+#define KZ_KDMP_HEADER_BYTES 44
 
 #endif /* KZ_TYPES_H */

@@ -24,7 +24,8 @@ FLAG_ENTROPY, FLAG_ENTROPY2, FLAG_ENTROPY3, FLAG_HD_FIXUP, FLAG_RAD_FIXUP, FLAG_
 GEOM_DOUBLES = 47  # kz_geom: gg[4][5] GG[4][5] gdet alpha gttpert xx[4]
 
 ERRORS = {0: "ok", -1: "SpacelikeVelocity", -2: "VelocityConversionFailed", -3: "NanInFlux",
-          -4: "OutOfMemory", -5: "InvalidConfig", -6: "HipError", -7: "Unsupported"}
+          -4: "OutOfMemory", -5: "InvalidConfig", -6: "HipError", -7: "Unsupported", -8: "BadMagic",
+          -9: "BadVersion", -10: "DimMismatch", -11: "Truncated"}
 
 
 class kz_floor_params(C.Structure):
@@ -61,6 +62,14 @@ class kz_opac_params(C.Structure):
         [(n, C.c_int32) for n in (
             "bremsstrahlung", "synchrotron", "kleinnishina", "comptonization",
             "synchrotron_bridge", "kappa_mode", "kappaes_mode", "_pad")]
+
+
+class kz_dump_header(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("nv", C.c_uint32), ("t", C.c_double),
+                ("nstep", C.c_uint64), ("out_idx", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+KDMP_HEADER_BYTES = 44
 
 
 class kz_radvisc_params(C.Structure):

@@ -2415,6 +2415,7 @@ int kz_sim_init_timestep_guess(kz_sim *s) {
 }
 
 int kz_sim_set_time(kz_sim *s, double t, double tstepdenmax, double tstepdenmin) {
+  HIPCHK(hipStreamSynchronize(s->stream));
   double tt = t;
   HIPCHK(hipMemcpy(s->d.ss + SS_T, &tt, sizeof tt, hipMemcpyHostToDevice));
   unsigned long long b[2] = {(unsigned long long)__builtin_bit_cast(long long, tstepdenmax),
@@ -2441,6 +2442,99 @@ int kz_sim_counters(kz_sim *s, kz_counters *c) {
   c->n_radimp_solves = cnt[2];
   c->n_guard_recoveries = cnt[3];
   return 0;
+}
+
+// ---- KDMP v2 checkpoints (koral/io/dump.zig:19-160) ----------------------
+// The header and body are little-endian; both hosts this library targets
+// (x86-64, aarch64) are, so fields are copied as they lie in memory.
+int kz_kdmp_parse_header(const uint8_t *buf, uint64_t len, kz_dump_header *h) {
+  if (len < KZ_KDMP_HEADER_BYTES) return KZ_E_TRUNCATED;
+  if (memcmp(buf, "KDMP", 4) != 0) return KZ_E_BAD_MAGIC;
+  uint32_t ver;
+  memcpy(&ver, buf + 4, 4);
+  if (ver != 2) return KZ_E_BAD_VERSION;
+  memcpy(&h->nx, buf + 8, 4);
+  memcpy(&h->ny, buf + 12, 4);
+  memcpy(&h->nz, buf + 16, 4);
+  memcpy(&h->nv, buf + 20, 4);
+  memcpy(&h->t, buf + 24, 8);
+  memcpy(&h->nstep, buf + 32, 8);
+  memcpy(&h->out_idx, buf + 40, 4);
+  h->_pad = 0;
+  return 0;
+}
+
+int kz_sim_kdmp_size(kz_sim *s, uint64_t *bytes) {
+  *bytes = KZ_KDMP_HEADER_BYTES + (uint64_t)s->g.nx * s->g.ny * s->g.nz * NV * 8;
+  return 0;
+}
+
+// serializePrimDump (dump.zig:101-135): header, then the domain primitives
+// p[iz][iy][ix][iv]; t and nstep are the run's current clock.
+int kz_sim_write_kdmp(kz_sim *s, uint32_t out_idx, uint8_t *buf, uint64_t cap, uint64_t *written) {
+  uint64_t need;
+  kz_sim_kdmp_size(s, &need);
+  if (cap < need) return KZ_E_TRUNCATED;
+  kz_counters c;
+  if (int e = kz_sim_counters(s, &c)) return e;
+  std::vector<double> p((size_t)s->g.ncell * NV);
+  if (int e = kz_sim_get_state(s, p.data(), nullptr, nullptr)) return e;
+  uint8_t *w = buf;
+  const uint32_t ver = 2, nv = NV, dims[3] = {(uint32_t)s->g.nx, (uint32_t)s->g.ny, (uint32_t)s->g.nz};
+  memcpy(w, "KDMP", 4);
+  memcpy(w + 4, &ver, 4);
+  memcpy(w + 8, dims, 12);
+  memcpy(w + 20, &nv, 4);
+  memcpy(w + 24, &c.t, 8);
+  memcpy(w + 32, &c.nstep, 8);
+  memcpy(w + 40, &out_idx, 4);
+  w += KZ_KDMP_HEADER_BYTES;
+  for (long iz = 0; iz < s->g.nz; ++iz)
+    for (long iy = 0; iy < s->g.ny; ++iy)
+      for (long ix = 0; ix < s->g.nx; ++ix) {
+        const long ci = (ix + s->g.ngx) + (iy + s->g.ngy) * (long)s->g.sx + (iz + s->g.ngz) * (long)s->g.sy * s->g.sx;
+        memcpy(w, p.data() + ci * NV, NV * 8);
+        w += NV * 8;
+      }
+  *written = (uint64_t)(w - buf);
+  return 0;
+}
+
+// loadPrimDump (dump.zig:137-160) + the driver's clock restore
+// (problems/puffy/main.zig:283-298): every domain cell gets its primitives
+// and u = p2u(p) (Sim.initCell); ghosts are left for kz_sim_finish_init.
+int kz_sim_load_kdmp(kz_sim *s, const uint8_t *buf, uint64_t len, kz_dump_header *h) {
+  kz_dump_header hh;
+  if (int e = kz_kdmp_parse_header(buf, len, &hh)) return e;
+  if (hh.nx != (uint32_t)s->g.nx || hh.ny != (uint32_t)s->g.ny || hh.nz != (uint32_t)s->g.nz || hh.nv != NV)
+    return KZ_E_DIM_MISMATCH;
+  uint64_t need;
+  kz_sim_kdmp_size(s, &need);
+  if (len < need) return KZ_E_TRUNCATED;
+  std::vector<double> p((size_t)s->g.ncell * NV);
+  if (int e = kz_sim_get_state(s, p.data(), nullptr, nullptr)) return e;
+  const uint8_t *r = buf + KZ_KDMP_HEADER_BYTES;
+  for (long iz = 0; iz < s->g.nz; ++iz)
+    for (long iy = 0; iy < s->g.ny; ++iy)
+      for (long ix = 0; ix < s->g.nx; ++ix) {
+        const long ci = (ix + s->g.ngx) + (iy + s->g.ngy) * (long)s->g.sx + (iz + s->g.ngz) * (long)s->g.sy * s->g.sx;
+        memcpy(p.data() + ci * NV, r, NV * 8);
+        r += NV * 8;
+      }
+  if (int e = kz_sim_set_state(s, p.data(), nullptr)) return e;
+  if (int e = kz_sim_p2u_domain(s)) return e;
+  double t = hh.t;
+  HIPCHK(hipMemcpyAsync(s->d.ss + SS_T, &t, sizeof t, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->nstep = hh.nstep;
+  if (h) *h = hh;
+  return 0;
+}
+
+// Sim.finishInit (koral/sim.zig:686-690): ghosts, then the startup dt guess.
+int kz_sim_finish_init(kz_sim *s) {
+  if (int e = kz_sim_set_bc(s, 1)) return e;
+  return kz_sim_init_timestep_guess(s);
 }
 
 int kz_sim_time_kernel(kz_sim *s, int kernel) {

@@ -47,6 +47,11 @@ _SIGS = {
     "kz_calc_rij_visc_total": (C.c_int, [C.c_void_p, C.c_double]),
     "kz_sim_get_rijvisc": (C.c_int, [C.c_void_p, D]),
     "kz_apply_dynamo": (C.c_int, [C.c_void_p, C.c_double]),
+    "kz_kdmp_parse_header": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(T.kz_dump_header)]),
+    "kz_sim_kdmp_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_sim_write_kdmp": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kz_sim_load_kdmp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(T.kz_dump_header)]),
+    "kz_sim_finish_init": (C.c_int, [C.c_void_p]),
     # kz_implicit.h
     "kz_implicit_batch_create": (C.c_void_p, [C.c_int64, C.POINTER(T.kz_physics), I32]),
     "kz_implicit_batch_destroy": (None, [C.c_void_p]),
@@ -88,6 +93,13 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def parse_kdmp_header(data):
+    h = T.kz_dump_header()
+    buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    _chk(lib().kz_kdmp_parse_header(buf, len(data), C.byref(h)), "kz_kdmp_parse_header")
+    return {k: getattr(h, k) for k, _ in h._fields_ if k != "_pad"}
 
 
 def _chk(rc, what):
@@ -217,6 +229,25 @@ class Sim:
 
     def apply_dynamo(self, dt):
         _chk(lib().kz_apply_dynamo(self.h, dt), "kz_apply_dynamo")
+
+    def write_kdmp(self, out_idx=0):
+        """KDMP v2 checkpoint of the run's domain primitives and clock (bytes)."""
+        n = C.c_uint64(0)
+        _chk(lib().kz_sim_kdmp_size(self.h, C.byref(n)), "kz_sim_kdmp_size")
+        buf = (C.c_uint8 * n.value)()
+        w = C.c_uint64(0)
+        _chk(lib().kz_sim_write_kdmp(self.h, out_idx, buf, n.value, C.byref(w)), "kz_sim_write_kdmp")
+        return bytes(buf[:w.value])
+
+    def load_kdmp(self, data, finish=True):
+        """Restart from a KDMP checkpoint (then ghosts + dt guess unless
+        finish=False); returns the header as a dict."""
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        h = T.kz_dump_header()
+        _chk(lib().kz_sim_load_kdmp(self.h, buf, len(data), C.byref(h)), "kz_sim_load_kdmp")
+        if finish:
+            _chk(lib().kz_sim_finish_init(self.h), "kz_sim_finish_init")
+        return {k: getattr(h, k) for k, _ in h._fields_ if k != "_pad"}
 
     def kernel_time(self):
         """(summed ms, launches) of the timed class since the last call."""

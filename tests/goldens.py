@@ -153,3 +153,20 @@ class DevTracker:
 
     def ok(self, bound):
         return self.max_dev <= bound
+
+
+def kdmp_bytes(p_dom, t, nstep, out_idx):
+    """io/dump.zig:101-135 serializePrimDump: the 44-byte v2 header then the
+    domain primitives p[nz][ny][nx][nv], little-endian."""
+    nz, ny, nx, nv = p_dom.shape
+    head = b"KDMP" + struct.pack("<IIIIIdQI", 2, nx, ny, nz, nv, t, nstep, out_idx)
+    return head + np.ascontiguousarray(p_dom, "<f8").tobytes()
+
+
+def read_kdmp(b):
+    """io/dump.zig:82-98 parseDumpHeader + the body."""
+    assert b[:4] == b"KDMP"
+    ver, nx, ny, nz, nv, t, nstep, out_idx = struct.unpack_from("<IIIIIdQI", b, 4)
+    assert ver == 2
+    p = np.frombuffer(b, "<f8", offset=44, count=nx * ny * nz * nv).reshape(nz, ny, nx, nv)
+    return dict(nx=nx, ny=ny, nz=nz, nv=nv, t=t, nstep=nstep, out_idx=out_idx, p=p)
