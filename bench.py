@@ -142,6 +142,75 @@ def bench_implicit_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_run_server(args):
+    """Many PUFFY runs served concurrently on one GPU: Poisson arrivals,
+    admission by device footprint against an HBM budget, per-run HIP-graph
+    batches of CFL steps on per-run streams.  value = zone-cycles of all runs
+    over the wall time from the first arrival to the last completion; the
+    same runs stepped one after another with the plain eager loop give the
+    sequential figure."""
+    from koral_zig_amd import abi_types as T
+    from koral_zig_amd import server
+
+    specs = server.poisson_specs(args.runs, args.rate, args.run_steps, args.seed)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    log(f"run server: {len(specs)} runs, preparing host set-up in {workers} threads")
+    prep = server.prepare(specs, workers, log=log)
+    log(f"host set-up {time.perf_counter() - t0:.1f}s (metric tables + prepinit, before the clock)")
+    seq_prep = {k: v for k, v in prep.items()}  # the sequential pass consumes its own copy
+    budget = int(args.hbm_gib * (1 << 30))
+    srv = server.RunServer(budget, batch_steps=args.batch, log=log)
+    res = srv.serve(specs, prep)
+    lat = res["latency_ms"]
+    value = res["zone_cycles"] / res["wall_s"]
+    log(f"served {res['runs']} runs in {res['wall_s']:.2f}s: {value:.4e} zone-cycles/s, peak resident "
+        f"{res['peak_resident']} ({res['peak_bytes'] / 2**30:.2f} GiB); step latency p50 "
+        f"{np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
+    seq = server.serve_sequential(specs, seq_prep, log=log)
+    seq_v = seq["zone_cycles"] / seq["step_s"]
+    log(f"sequential eager: stepping {seq['step_s']:.2f}s ({seq_v:.4e} zone-cycles/s), {seq['wall_s']:.2f}s "
+        f"with each run's init")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        g = T.grid_puffy(64, 64)
+        from koral_zig_amd import host
+        v, secs = cpu_baseline_fresh(g, T.physics_puffy(), host.MP_PUFFY, args.seed, False, args.cpu_steps)
+        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} CFL steps of one 64x64 run, a=0, from its own seeded init "
+                         f"({secs:.1f}s, one thread)"}
+        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+    sizes = "{" + ",".join(f"{a}x{b}" for a, b in server.C4_SIZES) + "}"
+    out = {
+        "metric": "aggregate zone-cycles/s of a PUFFY run server on 1 GPU; p50/p99 step latency",
+        "value": value,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": int(sum(s.nsteps for s in specs)),
+        "warmup": 0,
+        "ms_per_step": float(np.percentile(lat, 50)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"run server: {args.runs} PUFFY runs, Poisson arrivals {args.rate}/s, sizes "
+                               f"{sizes} uniform, spin U[0,0.95], {args.run_steps} CFL steps each, graph batches "
+                               f"of {args.batch} steps, HBM budget {args.hbm_gib} GiB",
+                   "seed": args.seed},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                            "n": int(lat.size), "definition": "graph batch launch to observed completion / steps"},
+        "server": {"wall_s": res["wall_s"], "zone_cycles": res["zone_cycles"], "peak_resident": res["peak_resident"],
+                   "peak_gib": res["peak_bytes"] / 2**30, "last_arrival_s": specs and max(s.arrival for s in specs)},
+        "sequential_eager": {"value": seq_v, "step_s": seq["step_s"], "wall_with_init_s": seq["wall_s"],
+                             "speedup": value / seq_v},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def face_units(grid):
     """Faces one k_face_flux launch computes (every active dimension)."""
     n = (grid.nx, grid.ny, grid.nz)
@@ -181,9 +250,14 @@ def main():
     ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--latency-steps", type=int, default=50)
-    ap.add_argument("--workload", choices=["puffy", "wedge3d", "implicit-batch"], default="puffy",
+    ap.add_argument("--workload", choices=["puffy", "wedge3d", "implicit-batch", "run-server"], default="puffy",
                     help="puffy: the 384x360 step (default); wedge3d: the 3D wedge; implicit-batch: the "
-                         "independent-cell solve batch")
+                         "independent-cell solve batch; run-server: many runs served concurrently")
+    ap.add_argument("--runs", type=int, default=96, help="run-server: number of runs")
+    ap.add_argument("--rate", type=float, default=4.0, help="run-server: Poisson arrival rate per second")
+    ap.add_argument("--run-steps", type=int, default=500, help="run-server: CFL steps per run")
+    ap.add_argument("--batch", type=int, default=10, help="run-server: steps per captured graph")
+    ap.add_argument("--hbm-gib", type=float, default=8.0, help="run-server: device-memory budget of resident runs")
     ap.add_argument("--nx3", type=int, default=192)
     ap.add_argument("--ny3", type=int, default=192)
     ap.add_argument("--nz3", type=int, default=64)
@@ -202,6 +276,9 @@ def main():
         if "--warmup" not in sys.argv:
             args.warmup = 1
         bench_implicit_batch(args)
+        return
+    if args.workload == "run-server":
+        bench_run_server(args)
         return
     if args.workload == "wedge3d":
         if "--steps" not in sys.argv:

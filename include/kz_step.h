@@ -90,6 +90,19 @@ int kz_step(kz_sim *s, const double *forced_dt);
  * returns the first device-side error. */
 int kz_step_many(kz_sim *s, int nsteps);
 int kz_sim_sync(kz_sim *s);
+/* Capture nsteps CFL steps into a HIP graph (the launches kz_step_many
+ * enqueues; event timing off) and replay it: kz_sim_replay enqueues the
+ * graph on the sim's stream without waiting and advances the run exactly
+ * like kz_step_many(nsteps).  For run servers that keep many runs resident
+ * and advance them in batches; no counterpart in the reference, whose steps
+ * are host-driven. */
+int kz_sim_capture_steps(kz_sim *s, int nsteps);
+int kz_sim_replay(kz_sim *s);
+/* 0 when everything enqueued on the sim's stream has finished, 1 while
+ * work is pending (a non-blocking poll; kz_sim_sync reports errors). */
+int kz_sim_query(kz_sim *s);
+/* Device memory this run holds (bytes), for admission by HBM footprint. */
+int kz_sim_device_bytes(kz_sim *s, uint64_t *bytes);
 /* The hipStream_t the sim enqueues on. */
 void *kz_sim_stream(kz_sim *s);
 

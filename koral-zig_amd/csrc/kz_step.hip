@@ -1875,6 +1875,11 @@ struct kz_sim {
   int timed = -1;
   std::vector<hipEvent_t> ev_pool;  // created events, reused
   size_t ev_used = 0;               // events recorded since the last read
+  // device bytes allocated for this run, and the captured multi-step graph
+  size_t dev_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
 };
 
 // Brackets the launches of one kernel class with events on the sim's stream
@@ -1916,6 +1921,7 @@ struct TimedLaunch {
 static int dalloc(kz_sim *s, void **p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) return KZ_E_OOM;
   s->allocs.push_back(*p);
+  s->dev_bytes += bytes;
   return hipMemset(*p, 0, bytes) == hipSuccess ? 0 : KZ_E_HIP;
 }
 
@@ -2080,6 +2086,8 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
 void kz_sim_destroy(kz_sim *s) {
   if (!s) return;
   (void)hipStreamSynchronize(s->stream);
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void *p : s->allocs) (void)hipFree(p);
   for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(s->stream);
@@ -2386,6 +2394,46 @@ int kz_step_many(kz_sim *s, int nsteps) {
 }
 
 int kz_sim_sync(kz_sim *s) { return check_err(s); }
+
+// Capture nsteps CFL steps (exactly the launches kz_step_many enqueues) into
+// a HIP graph; kz_sim_replay enqueues that graph.  Every step reads its dt
+// and clears its counters and lists with kernels on the device, so a replay
+// continues the run like kz_step_many(nsteps).  Event timing must be off.
+int kz_sim_capture_steps(kz_sim *s, int nsteps) {
+  if (nsteps <= 0 || s->timed != -1) return KZ_E_INVALID_CONFIG;
+  if ((s->ph.radviscosity || s->ph.dynamo) && !s->has_bl) return KZ_E_INVALID_CONFIG;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->graph_exec) HIPCHK(hipGraphExecDestroy(s->graph_exec));
+  if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+  s->graph_steps = 0;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < nsteps; ++i) launch_step(s, 0.0, 0);
+  HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
+  HIPCHK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+  s->graph_steps = nsteps;
+  return 0;
+}
+
+int kz_sim_replay(kz_sim *s) {
+  if (!s->graph_exec) return KZ_E_INVALID_CONFIG;
+  HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
+  s->nstep += (uint64_t)s->graph_steps;
+  return 0;
+}
+
+int kz_sim_query(kz_sim *s) {
+  hipError_t e = hipStreamQuery(s->stream);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  return KZ_E_HIP;
+}
+
+int kz_sim_device_bytes(kz_sim *s, uint64_t *bytes) {
+  *bytes = s->dev_bytes;
+  return 0;
+}
 
 void *kz_sim_stream(kz_sim *s) { return (void *)s->stream; }
 

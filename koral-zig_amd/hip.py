@@ -40,6 +40,10 @@ _SIGS = {
     "kz_step_many": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_sync": (C.c_int, [C.c_void_p]),
     "kz_sim_stream": (C.c_void_p, [C.c_void_p]),
+    "kz_sim_capture_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_sim_replay": (C.c_int, [C.c_void_p]),
+    "kz_sim_query": (C.c_int, [C.c_void_p]),
+    "kz_sim_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_sim_counters": (C.c_int, [C.c_void_p, C.POINTER(T.kz_counters)]),
     "kz_sim_time_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_kernel_time": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_uint64)]),
@@ -229,6 +233,26 @@ class Sim:
 
     def apply_dynamo(self, dt):
         _chk(lib().kz_apply_dynamo(self.h, dt), "kz_apply_dynamo")
+
+    def capture_steps(self, nsteps):
+        """Capture nsteps CFL steps into a HIP graph (see replay)."""
+        _chk(lib().kz_sim_capture_steps(self.h, int(nsteps)), "kz_sim_capture_steps")
+
+    def replay(self):
+        """Enqueue the captured steps without waiting (sync() waits)."""
+        _chk(lib().kz_sim_replay(self.h), "kz_sim_replay")
+
+    def busy(self):
+        """True while enqueued work is still running (non-blocking)."""
+        rc = lib().kz_sim_query(self.h)
+        if rc < 0:
+            _chk(rc, "kz_sim_query")
+        return rc == 1
+
+    def device_bytes(self):
+        n = C.c_uint64(0)
+        _chk(lib().kz_sim_device_bytes(self.h, C.byref(n)), "kz_sim_device_bytes")
+        return n.value
 
     def write_kdmp(self, out_idx=0):
         """KDMP v2 checkpoint of the run's domain primitives and clock (bytes)."""

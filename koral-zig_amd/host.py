@@ -133,15 +133,16 @@ def geometry_bl(grid, ix, iy, iz=0, mp=MP_PUFFY):
     return out
 
 
-def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-4, rho_u_only=False):
+def init_puffy(engine, grid, tables, phys, init=PUFFY_INIT, seed=None, amp=1.0e-4, rho_u_only=False, p0=None):
     """The ko.c init sequence (puffy.zig:873-884 initAllWith) driven through
     an engine (the HIP sim handle, or the CPU restatement in tests):
     prepinit+init -> set_bc -> calc_BfromA -> set_bc -> postinit (beta norm)
     -> initTimestepGuess; with `seed`, the seeded 1e-4 perturbation of every
     non-B primitive (of rho and u only with rho_u_only) is applied after
-    postinit, followed by p2u + set_bc.
+    postinit, followed by p2u + set_bc.  `p0` is puffy_prepinit's result when
+    the caller computed it already.
     Returns the beta-normalisation factor."""
-    p = puffy_prepinit(grid, tables, phys, init)
+    p = puffy_prepinit(grid, tables, phys, init) if p0 is None else p0
     engine.set_state(p=p)
     engine.p2u_domain()
     engine.set_bc(True)
