@@ -1034,8 +1034,11 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, D
 // implicit four-force solve (sim.zig:1736-1776 -> implicit.zig:927-1004),
 // split in three kernels so that each keeps its state in registers:
 //   k_imp_prep     reference state, bisection warm start, starting block
-//   k_imp_rung0<W> the first rung (energy equation, lab frame) for the
-//                  cells whose starting primitive block is W
+//   k_imp_rung0    the first rung (energy equation, lab frame), one launch
+//                  over both starting-block lists: the RAD-start cells (the
+//                  longer solves) take the first blocks, so their waves are
+//                  dispatched first and the MHD-start waves fill the SIMDs
+//                  that remain and those freed as waves finish
 //   k_imp_slow     rungs 1..5 (and damped-opacity levels) for the cells the
 //                  first rung did not solve, gathered into a list
 // The sequence of attempts per cell is the reference ladder's; inputs to
@@ -1184,9 +1187,8 @@ __device__ __forceinline__ void imp_load(const ImpWork &w, long nd, long t, doub
   lag.op.rad_ross = a[IA_KRR * nd + t];
 }
 template <class S, int WP>
-__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(S s, ImpWork w, kz_physics ph) {
+__device__ __forceinline__ void rung0_cell(const S &s, const ImpWork &w, const kz_physics &ph, long i) {
   constexpr int which = WP == WP_MHD ? 0 : 1;
-  long i = TID;
   if (i >= (long)w.ncnt[which]) return;
   const long nd = s.count();
   long t = w.list[which * nd + i];
@@ -1207,6 +1209,15 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(S s, ImpW
     unsigned long long k = atomicAdd(w.nslow, 1ULL);
     w.slow[k] = (int32_t)t;
   }
+}
+// launched with nblk(n, 64) + 1 blocks: ceil(nrad/64) + ceil(nmhd/64) never
+// exceeds that for nrad + nmhd <= n
+template <class S>
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(S s, ImpWork w, kz_physics ph) {
+  const long brad = ((long)w.ncnt[1] + 63) / 64;
+  const long b = blockIdx.x;
+  if (b < brad) rung0_cell<S, WP_RAD>(s, w, ph, b * 64 + threadIdx.x);
+  else rung0_cell<S, WP_MHD>(s, w, ph, (b - brad) * 64 + threadIdx.x);
 }
 template <class S>
 __global__ void __launch_bounds__(64) k_imp_slow(S s, ImpWork w, kz_physics ph) {
@@ -2261,8 +2272,7 @@ static void launch_op_implicit(kz_sim *s, int dt_slot) {
     const GridCells cells{s->g, s->d, dt_slot};
     const ImpWork w = imp_work(s->d);
     k_imp_prep<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
-    k_imp_rung0<GridCells, WP_MHD><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
-    k_imp_rung0<GridCells, WP_RAD><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
+    k_imp_rung0<GridCells><<<nb + 1, 64, 0, s->stream>>>(cells, w, s->ph);
     k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
   }
   if (s->ph.do_fixups && s->ph.do_radimp_fixups)
@@ -2728,8 +2738,7 @@ int kz_implicit_batch_solve(kz_implicit_batch *b, double *ms) {
   const unsigned nb = nblk(n, 64);
   HIPCHK(hipEventRecord(b->ev[0], b->stream));
   k_imp_prep<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
-  k_imp_rung0<BatchCells, WP_MHD><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
-  k_imp_rung0<BatchCells, WP_RAD><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_rung0<BatchCells><<<nb + 1, 64, 0, b->stream>>>(cells, b->w, b->ph);
   k_imp_slow<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
   HIPCHK(hipEventRecord(b->ev[1], b->stream));
   HIPCHK(hipGetLastError());
