@@ -40,12 +40,21 @@ FP64_PEAK_TFLOPS = 78.6
 #  u2p (per cell): read u 104 + p 104 + geometry 176 + write p 104
 #    + 4 flags 16                                               = 504 B
 BYTES_PER_UNIT = {"face_flux": 400.0, "update": 1224.0, "u2p": 504.0}
-# FP64 flops of one implicit cell-solve: an a-priori operation count of the
-# solver's fixed part plus its per-Newton-iteration part (Jacobian by 4
-# residual differences, 4x4 inverse, update); refined by an instrumented
-# count later.
-IMPLICIT_FLOPS_PER_SOLVE = 12000.0
-IMPLICIT_FLOPS_PER_ITER = 10000.0
+# FP64 operations of one implicit cell-solve, counted by the oracle built
+# with a counting double (oracle/flops: +1 per + - * / sqrt, transcendentals
+# counted through their own operations).  `python oracle/flops/count.py c1`
+# over the 136 704 implicit cells of the 384x360 step after its first step:
+# 31 617.5 per solve on average; least-squares fit over the cells
+# flops = 17 034.1 + 10 609.1 * Newton iterations (rms 2.4e3).  The bench
+# multiplies the fit by the solves and iterations the device counts.
+IMPLICIT_FLOPS_PER_SOLVE = 17034.1
+IMPLICIT_FLOPS_PER_ITER = 10609.1
+# The independent-cell batch spends most of its operations in the later
+# rungs and in failing ladders, which the fit above does not describe:
+# `python oracle/flops/count.py c2` over 200 000 cells of the same generator
+# counts 311 175.3 operations per cell (first-rung solves 91 938.7, later
+# rungs 720 573.7, failed cells 2 666 322.8).
+BATCH_FLOPS_PER_CELL = 311175.3
 # HBM bytes per launch of a kernel class at 384x360, from rocprofv3 runs of
 # FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
 # counter reads for a copy kernel of known size in the same run (k_soa2aos:
@@ -106,13 +115,51 @@ def bench_implicit_batch(args):
         kms.append(b.solve())
     wall = time.perf_counter() - t0
     _, _, res, (nfail, iters, solves) = b.get()
-    b.close()
     value = n * args.steps / wall
     avg_ms = float(np.mean(kms))
-    flops = IMPLICIT_FLOPS_PER_SOLVE * n + IMPLICIT_FLOPS_PER_ITER * iters
+    flops = BATCH_FLOPS_PER_CELL * n
     achieved = flops / (avg_ms * 1e-3) / 1e12
     log(f"timed {args.steps} solves in {wall:.3f}s: {value:.4e} cell-solves/s; kernels {avg_ms:.3f} ms/solve; "
         f"ok {solves}/{n}, {iters / max(solves, 1):.2f} iterations per solve")
+    mixed = None
+    if args.mixed:
+        # the same cells with the FP32-Jacobian Newton, against the FP64 result
+        _, p64, r64, _ = b.get()
+        b.close()
+        mp = T.physics_puffy()
+        mp.implicit.jacobian_fp32 = 1
+        b = hip.ImplicitBatch(n, mp)
+        b.set(pp, geom, dt)
+        for _ in range(max(args.warmup, 1)):
+            b.solve()
+        kms32 = []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            kms32.append(b.solve())
+        wall32 = time.perf_counter() - t1
+        _, p32, r32, (nfail32, iters32, solves32) = b.get()
+        ok64, ok32 = r64["ok"] == 1, r32["ok"] == 1
+        both = ok64 & ok32 & (r64["rung"] == r32["rung"])
+        rel = np.abs(p64[both] - p32[both]) / np.maximum(np.maximum(np.abs(p64[both]), np.abs(p32[both])), 1e-300)
+        cell_max = rel.max(1)
+        v32 = n * args.steps / wall32
+        mixed = {
+            "value": v32, "unit": "cell-solves/s", "speedup": v32 / value, "ms_per_solve": float(np.mean(kms32)),
+            "ok_identical": float((ok64 == ok32).mean()), "rung_identical": float((r64["rung"] == r32["rung"]).mean()),
+            "iters_identical": float((r64["iters"] == r32["iters"]).mean()),
+            "iterations_per_solve": iters32 / max(solves32, 1), "failures": nfail32,
+            "field_rel_err_vs_fp64": {
+                "cells": int(both.sum()),
+                "max_per_field": [float(x) for x in rel.max(0)],
+                "p99_per_field": [float(x) for x in np.percentile(rel, 99, axis=0)],
+                "cell_max_median": float(np.median(cell_max)),
+                "frac_cells_le_1e-10": float((cell_max <= 1e-10).mean()),
+                "frac_cells_le_1e-6": float((cell_max <= 1e-6).mean())},
+        }
+        log(f"FP32-Jacobian: {v32:.4e} cell-solves/s ({v32 / value:.3f}x), ok identical {mixed['ok_identical']:.4f}, "
+            f"rung identical {mixed['rung_identical']:.4f}, median per-cell max rel err "
+            f"{mixed['field_rel_err_vs_fp64']['cell_max_median']:.1e}")
+    b.close()
     cpu = None
     if args.cpu_cells > 0:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -136,6 +183,7 @@ def bench_implicit_batch(args):
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "implicit",
                      "avg_launch_ms": avg_ms, "launches": len(kms)},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
         "solver": {"failures": nfail, "iterations": iters, "solves": solves,
                    "rungs": np.bincount(res["rung"][res["ok"] == 1], minlength=6).tolist()},
     }
@@ -263,6 +311,8 @@ def main():
     ap.add_argument("--nz3", type=int, default=64)
     ap.add_argument("--cells", type=int, default=1 << 22, help="implicit-batch: number of cells")
     ap.add_argument("--cpu-cells", type=int, default=20000, help="implicit-batch: oracle sample size (0: skip)")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
+                    help="implicit-batch: skip the FP32-Jacobian comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs the step on one GPU only; --gpus must be 1 "

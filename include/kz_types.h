@@ -105,7 +105,11 @@ typedef struct kz_implicit_params {
   int32_t allow_entr_in_4dprim;
   int32_t opdamp_maxlevels;
   int32_t lag_opac;
-  int32_t _pad;
+  /* Not in the reference: nonzero rounds the finite-difference Jacobian's
+   * quotients to FP32 and scales and LU-factors it in FP32 (the residuals,
+   * the constraint inversions and the Newton update stay FP64); 0 is the
+   * reference's all-FP64 solve. */
+  int32_t jacobian_fp32;
 } kz_implicit_params;
 
 /* Opacity / four-force constants: thermo.Consts (koral/physics/thermo.zig:

@@ -47,7 +47,7 @@ class kz_implicit_params(C.Structure):
         "max_tgas_change", "max_trad_change", "threshold", "opdamp_factor")] + \
         [(n, C.c_int32) for n in (
             "maxiter", "start_with_bisect", "scale_jacobian", "allow_rad_ceiling",
-            "allow_entr_in_4dprim", "opdamp_maxlevels", "lag_opac", "_pad")]
+            "allow_entr_in_4dprim", "opdamp_maxlevels", "lag_opac", "jacobian_fp32")]
 
 
 class kz_opac_params(C.Structure):

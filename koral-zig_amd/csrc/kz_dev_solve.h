@@ -468,8 +468,13 @@ KZD int check_floors_rad(double *pp, const Geo &gm, const kz_rad_params &rp) {
 // ---- implicit.zig --------------------------------------------------------------
 KZD double my_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 1.0); }  // :167-172
 
-KZD void invert4(const double a[4][4], double inv[4][4]) {  // :176-224
-  double lu[4][4];
+KZD double abs_t(double x) { return kzm_fabs(x); }
+KZD float abs_t(float x) { return __builtin_fabsf(x); }
+// implicit.zig:176-224 invert4, on doubles (the reference) or on floats (the
+// mixed-precision Newton's LU; same pivoting and elimination order)
+template <class T>
+KZD void invert4_t(const T a[4][4], T inv[4][4]) {
+  T lu[4][4];
   int perm[4] = {0, 1, 2, 3};
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -478,11 +483,11 @@ KZD void invert4(const double a[4][4], double inv[4][4]) {  // :176-224
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int piv = k;
-    double pmax = kzm_fabs(lu[k][k]);
+    T pmax = abs_t(lu[k][k]);
 #pragma unroll
     for (int r = k + 1; r < 4; ++r)
-      if (kzm_fabs(lu[r][k]) > pmax) {
-        pmax = kzm_fabs(lu[r][k]);
+      if (abs_t(lu[r][k]) > pmax) {
+        pmax = abs_t(lu[r][k]);
         piv = r;
       }
     // row swap expressed as selects so the matrix stays in registers
@@ -491,7 +496,7 @@ KZD void invert4(const double a[4][4], double inv[4][4]) {  // :176-224
       bool sw = piv == r;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        double tk = lu[k][j], tr = lu[r][j];
+        T tk = lu[k][j], tr = lu[r][j];
         lu[k][j] = sw ? tr : tk;
         lu[r][j] = sw ? tk : tr;
       }
@@ -508,10 +513,10 @@ KZD void invert4(const double a[4][4], double inv[4][4]) {  // :176-224
   }
 #pragma unroll
   for (int col = 0; col < 4; ++col) {
-    double y[4], x[4];
+    T y[4], x[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      y[r] = (perm[r] == col) ? 1.0 : 0.0;
+      y[r] = (perm[r] == col) ? T(1) : T(0);
 #pragma unroll
       for (int cc = 0; cc < r; ++cc) y[r] -= lu[r][cc] * y[cc];
     }
@@ -526,6 +531,7 @@ KZD void invert4(const double a[4][4], double inv[4][4]) {  // :176-224
     for (int rr = 0; rr < 4; ++rr) inv[rr][col] = x[rr];
   }
 }
+KZD void invert4(const double a[4][4], double inv[4][4]) { invert4_t<double>(a, inv); }
 
 struct ImpCtx {
   const Geo *gm;
@@ -822,14 +828,38 @@ __device__ static __forceinline__ bool solve_4d_prim_core(const double *uu00, co
         if (sv[j] == 0.0) sv[j] = 1.0;
         isv[j] = 1.0 / sv[j];
       }
+    }
+    double ij[4][4];
+    if (ip.jacobian_fp32) {
+      // mixed precision: FP64 quotients rounded to FP32, scaled and
+      // factored in FP32, the inverse applied in FP64
+      float jf[4][4], ijf[4][4];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (i != j) jac[i][j] = jac[i][j] * (sv[j] * isv[i]);
+        for (int j = 0; j < 4; ++j) jf[i][j] = (float)jac[i][j];
+      if (ip.scale_jacobian) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (i != j) jf[i][j] = jf[i][j] * ((float)sv[j] * (float)isv[i]);
+      }
+      invert4_t<float>(jf, ijf);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ij[i][j] = (double)ijf[i][j];
+    } else {
+      if (ip.scale_jacobian) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (i != j) jac[i][j] = jac[i][j] * (sv[j] * isv[i]);
+      }
+      invert4(jac, ij);
     }
-    double ij[4][4];
-    invert4(jac, ij);
     double xiapp = 1.0;
     double xxx[4];
     for (;;) {

@@ -142,6 +142,7 @@ int kzo_check_floors_rad(double *pp, const kzo_geo *gm, const kz_rad_params *rp)
 
 /* ---- solve/implicit.zig ---- */
 void kzo_invert4(const double a[4][4], double inv[4][4]);
+void kzo_invert4f(const float a[4][4], float inv[4][4]);
 kz_implicit_result kzo_solve_implicit_lab(double *uu, double *pp, const kzo_geo *gm, double dt,
                                           double gam, const kz_rad_params *rp,
                                           const kz_opac_params *op, const kz_implicit_params *ip);

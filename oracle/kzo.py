@@ -268,6 +268,13 @@ def invert4(a16):
     return out
 
 
+def invert4f(a16):
+    a16 = np.ascontiguousarray(a16, np.float32).reshape(-1, 16)
+    out = np.zeros_like(a16)
+    lib().kzo_invert4f_batch(C.c_int64(len(a16)), T.ptr(a16, C.c_float), T.ptr(out, C.c_float))
+    return out
+
+
 MATH_FN = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "log1p": 4, "sqrt": 5, "div": 6}
 
 

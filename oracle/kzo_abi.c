@@ -177,6 +177,11 @@ int kzo_invert4_batch(int64_t n, const double *a16, double *inv16) {
   return 0;
 }
 
+int kzo_invert4f_batch(int64_t n, const float *a16, float *inv16) {
+  for (int64_t i = 0; i < n; ++i) kzo_invert4f((const float(*)[4])(a16 + i * 16), (float(*)[4])(inv16 + i * 16));
+  return 0;
+}
+
 /* fn: 0 exp 1 log 2 pow 3 cbrt 4 log1p 5 sqrt 6 div */
 int kzo_math_batch(int64_t n, int32_t fn, const double *x, const double *y, double *out) {
   for (int64_t i = 0; i < n; ++i) {

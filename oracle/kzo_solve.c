@@ -578,6 +578,52 @@ void kzo_invert4(const double a[4][4], double inv[4][4]) {
   }
 }
 
+/* kzo_invert4 in FP32 (the mixed-precision Newton's LU, not in the
+ * reference): the same pivoting and elimination order on floats. */
+void kzo_invert4f(const float a[4][4], float inv[4][4]) {
+  float lu[4][4];
+  int perm[4] = {0, 1, 2, 3};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) lu[i][j] = a[i][j];
+  for (int k = 0; k < 4; ++k) {
+    int piv = k;
+    float pmax = __builtin_fabsf(lu[k][k]);
+    for (int r = k + 1; r < 4; ++r) {
+      if (__builtin_fabsf(lu[r][k]) > pmax) {
+        pmax = __builtin_fabsf(lu[r][k]);
+        piv = r;
+      }
+    }
+    if (piv != k) {
+      for (int j = 0; j < 4; ++j) {
+        float t = lu[k][j];
+        lu[k][j] = lu[piv][j];
+        lu[piv][j] = t;
+      }
+      int tp = perm[k];
+      perm[k] = perm[piv];
+      perm[piv] = tp;
+    }
+    for (int r = k + 1; r < 4; ++r) {
+      lu[r][k] /= lu[k][k];
+      for (int cc = k + 1; cc < 4; ++cc) lu[r][cc] -= lu[r][k] * lu[k][cc];
+    }
+  }
+  for (int col = 0; col < 4; ++col) {
+    float y[4], x[4];
+    for (int r = 0; r < 4; ++r) {
+      y[r] = (perm[r] == col) ? 1.0f : 0.0f;
+      for (int cc = 0; cc < r; ++cc) y[r] -= lu[r][cc] * y[cc];
+    }
+    for (int r = 3; r >= 0; --r) {
+      x[r] = y[r];
+      for (int cc = r + 1; cc < 4; ++cc) x[r] -= lu[r][cc] * x[cc];
+      x[r] /= lu[r][r];
+    }
+    for (int rr = 0; rr < 4; ++rr) inv[rr][col] = x[rr];
+  }
+}
+
 typedef struct {
   const kzo_geo *gm;
   double gam;
@@ -875,12 +921,28 @@ static int solve_4d_prim_core(const double *uu00, const kzo_radstate *state00, c
         if (scalevec[j] == 0.0) scalevec[j] = 1.0;
         invscalevec[j] = 1.0 / scalevec[j];
       }
-      for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j)
-          if (i != j) jac[i][j] = jac[i][j] * (scalevec[j] * invscalevec[i]);
     }
     double ijac[4][4];
-    kzo_invert4(jac, ijac);
+    if (ip->jacobian_fp32) {
+      /* mixed precision: FP64 quotients rounded to FP32, scaled and
+       * factored in FP32, the inverse applied in FP64 */
+      float jf[4][4], ijf[4][4];
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) jf[i][j] = (float)jac[i][j];
+      if (ip->scale_jacobian)
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j)
+            if (i != j) jf[i][j] = jf[i][j] * ((float)scalevec[j] * (float)invscalevec[i]);
+      kzo_invert4f(jf, ijf);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) ijac[i][j] = (double)ijf[i][j];
+    } else {
+      if (ip->scale_jacobian)
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j)
+            if (i != j) jac[i][j] = jac[i][j] * (scalevec[j] * invscalevec[i]);
+      kzo_invert4(jac, ijac);
+    }
 
     double xiapp = 1.0;
     for (;;) {

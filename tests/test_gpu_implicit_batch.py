@@ -83,3 +83,28 @@ def test_batch_repeat_deterministic(cells):
     assert c1 == c2 and same(p1, p2) and same(u1, u2)
     for k in FIELDS:
         assert same(r1[k], r2[k])
+
+
+def test_fp32_jacobian_matches_oracle(cells):
+    """The mixed-precision Newton (jacobian_fp32: FP32 quotients, scaling and
+    LU; FP64 residuals, inversions and update) on the device equals the
+    oracle's restatement of it bit for bit; it is a different solve from the
+    all-FP64 one, which it must still agree with on most outcomes."""
+    phys, pp, geom, dt = cells
+    mp = T.physics_puffy()
+    mp.implicit.jacobian_fp32 = 1
+    rc, pc, uc = kzo.solve_implicit(pp, geom, dt, mp.gam, mp.rad, mp.opac, mp.implicit)
+    b = hip.ImplicitBatch(len(pp), mp)
+    b.set(pp, geom, dt)
+    b.solve()
+    ug, pg, rg, (nfail, iters, solves) = b.get()
+    b.close()
+    for k in FIELDS:
+        bad = np.nonzero(rc[k] != rg[k])[0]
+        assert bad.size == 0, (k, bad[:5], rc[k][bad[:5]], rg[k][bad[:5]])
+    ok = rc["ok"] == 1
+    assert same(pc[ok], pg[ok]) and same(uc[ok], ug[ok])
+    r64, _, _ = kzo.solve_implicit(pp, geom, dt, phys.gam, phys.rad, phys.opac, phys.implicit)
+    assert (r64["ok"] == rc["ok"]).mean() > 0.99
+    assert (r64["rung"] == rc["rung"]).mean() > 0.95
+    assert not np.array_equal(r64["iters"], rc["iters"])  # the FP32 path is really taken
