@@ -77,11 +77,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(grid, phys, tab, p, u, clock, steps):
-    """The oracle (single thread) over `steps` CFL steps from the GPU's
-    post-warmup state: zone-cycles/s on this host."""
+def host_threads():
+    """Threads for the all-cores CPU baseline: OMP_NUM_THREADS when set (the
+    host's CPU share), else the CPUs this process may run on."""
+    n = os.environ.get("OMP_NUM_THREADS")
+    if n and n.isdigit() and int(n) > 0:
+        return int(n)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(grid, phys, tab, p, u, clock, steps, threads=1):
+    """The oracle over `steps` CFL steps from the GPU's post-warmup state on
+    `threads` host threads (bitwise the serial result): zone-cycles/s."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import kzo
+    kzo.set_threads(threads)
     sim = kzo.Sim(grid, phys, tab)
     sim.set_state(p, u)
     sim.set_time(clock["t"], clock["tstepdenmax"], clock["tstepdenmin"])
@@ -89,7 +99,8 @@ def cpu_baseline(grid, phys, tab, p, u, clock, steps):
     for _ in range(steps):
         sim.step()
     dt = time.perf_counter() - t0
-    return grid.nx * grid.ny * steps / dt, dt
+    kzo.set_threads(1)
+    return grid.nx * grid.ny * grid.nz * steps / dt, dt
 
 
 def bench_implicit_batch(args):
@@ -222,13 +233,13 @@ def bench_run_server(args):
 
     cpu = None
     if args.cpu_steps > 0:
-        g = T.grid_puffy(64, 64)
+        g = T.grid_puffy(128, 96)
         from koral_zig_amd import host
-        v, secs = cpu_baseline_fresh(g, T.physics_puffy(), host.MP_PUFFY, args.seed, False, args.cpu_steps)
-        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} CFL steps of one 64x64 run, a=0, from its own seeded init "
-                         f"({secs:.1f}s, one thread)"}
-        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+        nt = host_threads()
+        k1, kt = 3 * args.cpu_steps, 12 * args.cpu_steps
+        v1, s1 = cpu_baseline_fresh(g, T.physics_puffy(), host.MP_PUFFY, args.seed, False, k1)
+        vt, st = cpu_baseline_fresh(g, T.physics_puffy(), host.MP_PUFFY, args.seed, False, kt, nt)
+        cpu = cpu_entry(v1, s1, k1, vt, st, kt, nt, "one 128x96 run, a=0, from its own seeded init")
     sizes = "{" + ",".join(f"{a}x{b}" for a, b in server.C4_SIZES) + "}"
     out = {
         "metric": "aggregate zone-cycles/s of a PUFFY run server on 1 GPU; p50/p99 step latency",
@@ -271,13 +282,14 @@ def face_units(grid):
     return tot
 
 
-def cpu_baseline_fresh(grid, phys, mp, seed, rho_u_only, steps):
-    """The oracle (single thread) over `steps` CFL steps of `grid` from its
-    own seeded init: zone-cycles/s on this host."""
+def cpu_baseline_fresh(grid, phys, mp, seed, rho_u_only, steps, threads=1):
+    """The oracle over `steps` CFL steps of `grid` from its own seeded init on
+    `threads` host threads: zone-cycles/s on this host."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import kzo
     from koral_zig_amd import host
     tab = host.metric_tables(grid, mp)
+    kzo.set_threads(threads)
     sim = kzo.Sim(grid, phys, tab)
     host.init_puffy(sim, grid, tab, phys, seed=seed, rho_u_only=rho_u_only)
     sim.step()  # the startup step (tiny dt) is not timed
@@ -285,7 +297,18 @@ def cpu_baseline_fresh(grid, phys, mp, seed, rho_u_only, steps):
     for _ in range(steps):
         sim.step()
     dt = time.perf_counter() - t0
+    kzo.set_threads(1)
     return grid.nx * grid.ny * grid.nz * steps / dt, dt
+
+
+def cpu_entry(v1, s1, steps1, vt, st, stepst, nt, what):
+    """The cpu_baseline object: the all-cores figure as `value`, the
+    single-thread one beside it."""
+    log(f"cpu baseline: 1 thread {v1:.4e} zone-cycles/s ({s1:.1f}s), {nt} threads {vt:.4e} ({st:.1f}s)")
+    return {"value": vt, "unit": "zone-cycles/s", "cores": nt, "kind": "port",
+            "sample": f"{stepst} CFL steps of {what} on {nt} threads ({st:.1f}s); the same oracle, bitwise the "
+                      f"serial result, at 1 thread over {steps1} steps ({s1:.1f}s): {v1:.4e} zone-cycles/s",
+            "value_1thread": v1}
 
 
 def main():
@@ -437,19 +460,20 @@ def run_grid(args, grid, phys, mp, metric, label, rho_u_only, cpu_grid):
     lat = np.array(lat)
 
     cpu = None
+    nt = host_threads()
+    kt = max(args.cpu_steps, 4)
     if args.cpu_steps > 0 and cpu_grid is None:
         p, u, _ = sim.get_state()
-        v, secs = cpu_baseline(grid, phys, tab, p, u, sim.counters(), args.cpu_steps)
-        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} CFL steps of the same {grid.nx}x{grid.ny}x{grid.nz} run from the "
-                         f"GPU's post-warmup state ({secs:.1f}s, one thread)"}
-        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+        c = sim.counters()
+        v1, s1 = cpu_baseline(grid, phys, tab, p, u, c, args.cpu_steps)
+        vt, st = cpu_baseline(grid, phys, tab, p, u, c, kt, nt)
+        cpu = cpu_entry(v1, s1, args.cpu_steps, vt, st, kt, nt,
+                        f"the same {grid.nx}x{grid.ny}x{grid.nz} run from the GPU's post-warmup state")
     elif args.cpu_steps > 0:
-        v, secs = cpu_baseline_fresh(cpu_grid, phys, mp, args.seed, rho_u_only, args.cpu_steps)
-        cpu = {"value": v, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} CFL steps of a {cpu_grid.nx}x{cpu_grid.ny}x{cpu_grid.nz} run of the "
-                         f"same physics from its own seeded init ({secs:.1f}s, one thread)"}
-        log(f"cpu baseline {v:.4e} zone-cycles/s over {secs:.1f}s")
+        v1, s1 = cpu_baseline_fresh(cpu_grid, phys, mp, args.seed, rho_u_only, args.cpu_steps)
+        vt, st = cpu_baseline_fresh(cpu_grid, phys, mp, args.seed, rho_u_only, kt, nt)
+        cpu = cpu_entry(v1, s1, args.cpu_steps, vt, st, kt, nt,
+                        f"a {cpu_grid.nx}x{cpu_grid.ny}x{cpu_grid.nz} run of the same physics from its own seeded init")
     sim.close()
 
     out = {

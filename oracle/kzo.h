@@ -76,6 +76,10 @@ enum { WP_RAD = 1, WP_MHD = 2 };
 enum { WE_ENERGY = 0, WE_ENTROPY = 1 };
 enum { WF_LAB = 0, WF_FF = 1 };
 
+/* threads of the oracle sim's cell-parallel passes (results independent of it) */
+void kzo_set_threads(int n);
+int kzo_get_threads(void);
+
 /* ---- metric.zig:20-32 ---- */
 double kzo_r_horizon_bl(double a);
 double kzo_r_isco_bl(double a);

@@ -105,6 +105,11 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """Threads of the oracle sim's cell-parallel passes (bitwise the serial result)."""
+    lib().kzo_set_threads(int(n))
+
+
 def _geoms(g):
     g = np.ascontiguousarray(g, dtype=np.float64)
     assert g.ndim == 2 and g.shape[1] == T.GEOM_DOUBLES

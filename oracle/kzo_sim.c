@@ -14,6 +14,15 @@
 #include <string.h>
 #include "kzo.h"
 
+/* Threads of the cell-parallel passes (wavespeeds, sweeps, face fluxes,
+ * update, u2p, implicit): rows are dealt out dynamically, every cell and face
+ * is computed by one thread exactly as the serial loop computes it, and the
+ * only cross-cell results are max/min and integer sums, so any thread count
+ * gives the serial result bit for bit.  Default 1. */
+static int kzo_nthreads = 1;
+void kzo_set_threads(int n) { kzo_nthreads = n < 1 ? 1 : n; }
+int kzo_get_threads(void) { return kzo_nthreads; }
+
 #define NSCAL 20
 enum {
   S_AHDXL, S_AHDXR, S_AHDX, S_ARADXL, S_ARADXR, S_ARADX,
@@ -478,9 +487,11 @@ static int calc_wavespeeds(kzo_sim *s) {
   int active[3] = {s->grid.nx > 1, s->grid.ny > 1, s->grid.nz > 1};
   long lx = active[0] ? 1 : 0, ly = active[1] ? 1 : 0, lz = active[2] ? 1 : 0;
   double tmax = -1.0 / 0.0, tmin = 1.0 / 0.0;
-  for (long iz = -lz; iz < s->grid.nz + lz; ++iz)
-    for (long iy = -ly; iy < s->grid.ny + ly; ++iy)
+  const long wy = s->grid.ny + 2 * ly, nrow = (s->grid.nz + 2 * lz) * wy;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1) reduction(max : tmax) reduction(min : tmin)
+  for (long row = 0; row < nrow; ++row)
       for (long ix = -lx; ix < s->grid.nx + lx; ++ix) {
+        long iz = row / wy - lz, iy = row % wy - ly;
         const double *pp = cell(s->p, s, ix, iy, iz);
         kzo_geo g;
         geo_c(s, ix, iy, iz, &g);
@@ -539,8 +550,11 @@ static void save_timesteps(kzo_sim *s) {
 
 /* ---- u2p pass + fixups (sim.zig:924-1133) ---- */
 static void u2p_rows(kzo_sim *s) {
-  for (long iz = 0; iz < s->grid.nz; ++iz)
-  for (long iy = 0; iy < s->grid.ny; ++iy) {
+  const long ny = s->grid.ny, nrow = s->grid.nz * ny;
+  uint64_t nguard = 0;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1) reduction(+ : nguard)
+  for (long row = 0; row < nrow; ++row) {
+    long iz = row / ny, iy = row % ny;
     int polar = polar_row(s, iy);
     for (long ix = 0; ix < s->grid.nx; ++ix) {
       double uu[NV], pp[NV];
@@ -561,13 +575,14 @@ static void u2p_rows(kzo_sim *s) {
       kz_u2p_result r = kzo_u2p_mhd(uu, pp, &g, s->ph.gam, &s->ph.floors);
       if (r.entropy_used) *flagp(s, KZ_FLAG_ENTROPY, ix, iy, iz) = 1;
       kz_u2prad_result rr = kzo_u2p_rad(uu, pp, &g, &s->ph.rad);
-      kzo_check_floors_mhd(pp, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+      kzo_check_floors_mhd(pp, &g, s->ph.gam, &s->ph.floors, &nguard);
       kzo_check_floors_rad(pp, &g, &s->ph.rad);
       memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
       *flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, iz) = r.fixup ? 1 : 0;
       *flagp(s, KZ_FLAG_RAD_FIXUP, ix, iy, iz) = rr.corrected ? -1 : 0;
     }
   }
+  s->n_guard += nguard;
 }
 
 static void cell_fixup(kzo_sim *s, int which) {
@@ -1071,7 +1086,7 @@ static int apply_dynamo(kzo_sim *s, double dt) {
 static long cross_lo(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? -1 : 0; }
 static long cross_hi(const kzo_sim *s, int dim) { return ndim(s, dim) > 1 ? ndim(s, dim) + 1 : ndim(s, dim); }
 
-static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
+static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3], uint64_t *nguard) {
   long n = ndim(s, dim);
   int dol = i >= 0, dor = i < n;
   double st[5][NV];
@@ -1091,7 +1106,7 @@ static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
   if (dol) {
     kzo_geo g;
     geo_f(s, cellc[0], cellc[1], cellc[2], dim, &g);
-    kzo_check_floors_mhd(pl, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+    kzo_check_floors_mhd(pl, &g, s->ph.gam, &s->ph.floors, nguard);
     int e = kzo_f_flux_prime(pl, dim, &g, s->ph.gam, ffl);
     if (e) return e;
     if (s->ph.radviscosity) add_rad_visc_flux(s, dim, cellc, pl, &g, ffl); /* sim.zig:1349-1352 */
@@ -1101,7 +1116,7 @@ static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3]) {
   if (dor) {
     kzo_geo g;
     geo_f(s, cp[0], cp[1], cp[2], dim, &g);
-    kzo_check_floors_mhd(pr, &g, s->ph.gam, &s->ph.floors, &s->n_guard);
+    kzo_check_floors_mhd(pr, &g, s->ph.gam, &s->ph.floors, nguard);
     int e = kzo_f_flux_prime(pr, dim, &g, s->ph.gam, ffr);
     if (e) return e;
     if (s->ph.radviscosity) add_rad_visc_flux(s, dim, cp, pr, &g, ffr); /* sim.zig:1361-1364 */
@@ -1117,15 +1132,31 @@ static int sweep(kzo_sim *s, int dim) {
   long n = ndim(s, dim);
   if (n <= 1) return KZO_OK;
   int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
-  for (long c0 = cross_lo(s, c0d); c0 < cross_hi(s, c0d); ++c0)
-    for (long c1 = cross_lo(s, c1d); c1 < cross_hi(s, c1d); ++c1)
-      for (long i = -1; i < n + 1; ++i) {
-        long cl[3];
-        cl[dim] = i; cl[c0d] = c0; cl[c1d] = c1;
-        int e = sweep_face(s, dim, i, cl);
-        if (e) return e;
+  const long lo0 = cross_lo(s, c0d), lo1 = cross_lo(s, c1d);
+  const long w1 = cross_hi(s, c1d) - lo1, nline = (cross_hi(s, c0d) - lo0) * w1;
+  uint64_t nguard = 0;
+  int err = KZO_OK;
+  /* every line runs to its end; the first error in line order is reported */
+  long err_line = nline;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1) reduction(+ : nguard)
+  for (long line = 0; line < nline; ++line) {
+    long c0 = lo0 + line / w1, c1 = lo1 + line % w1;
+    for (long i = -1; i < n + 1; ++i) {
+      long cl[3];
+      cl[dim] = i; cl[c0d] = c0; cl[c1d] = c1;
+      int e = sweep_face(s, dim, i, cl, &nguard);
+      if (e) {
+#pragma omp critical(kzo_sweep_err)
+        if (line < err_line) {
+          err_line = line;
+          err = e;
+        }
+        break;
       }
-  return KZO_OK;
+    }
+  }
+  s->n_guard += nguard;
+  return err;
 }
 
 static int flux_face(kzo_sim *s, int dim, const long cf[3]) {
@@ -1158,13 +1189,17 @@ static int fluxes_at_faces(kzo_sim *s) {
     long n = ndim(s, dim);
     if (n <= 1) continue;
     int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
-    for (long c0 = cross_lo(s, c0d); c0 < cross_hi(s, c0d); ++c0)
-      for (long c1 = cross_lo(s, c1d); c1 < cross_hi(s, c1d); ++c1)
-        for (long i = 0; i <= n; ++i) {
-          long cf[3];
-          cf[dim] = i; cf[c0d] = c0; cf[c1d] = c1;
-          flux_face(s, dim, cf);
-        }
+    const long lo0 = cross_lo(s, c0d), lo1 = cross_lo(s, c1d);
+    const long w1 = cross_hi(s, c1d) - lo1, nline = (cross_hi(s, c0d) - lo0) * w1;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1)
+    for (long line = 0; line < nline; ++line) {
+      long c0 = lo0 + line / w1, c1 = lo1 + line % w1;
+      for (long i = 0; i <= n; ++i) {
+        long cf[3];
+        cf[dim] = i; cf[c0d] = c0; cf[c1d] = c1;
+        flux_face(s, dim, cf);
+      }
+    }
   }
   return KZO_OK;
 }
@@ -1231,9 +1266,11 @@ void kzo_flux_ct(kzo_sim *s) {
 }
 
 static void update_rows(kzo_sim *s, double dtin) {
-  for (long iz = 0; iz < s->grid.nz; ++iz)
-  for (long iy = 0; iy < s->grid.ny; ++iy)
+  const long ny = s->grid.ny, nrow = s->grid.nz * ny;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1)
+  for (long row = 0; row < nrow; ++row)
     for (long ix = 0; ix < s->grid.nx; ++ix) {
+      long iz = row / ny, iy = row % ny;
       double ms[NV];
       metric_source(s, ix, iy, iz, ms);
       double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, iz, 2);
@@ -1266,8 +1303,11 @@ int kzo_sim_op_explicit(kzo_sim *s, double dt) {
 /* ---- implicit operator (sim.zig:1715-1776) ---- */
 int kzo_sim_op_implicit(kzo_sim *s, double dtin) {
   if (!s->ph.has_opac) return KZO_OK;
-  for (long iz = 0; iz < s->grid.nz; ++iz)
-  for (long iy = 0; iy < s->grid.ny; ++iy) {
+  const long ny = s->grid.ny, nrow = s->grid.nz * ny;
+  uint64_t n_iters = 0, n_solves = 0, n_fail = 0;
+#pragma omp parallel for num_threads(kzo_nthreads) schedule(dynamic, 1) reduction(+ : n_iters, n_solves, n_fail)
+  for (long row = 0; row < nrow; ++row) {
+    long iz = row / ny, iy = row % ny;
     if (polar_row(s, iy)) continue;
     for (long ix = 0; ix < s->grid.nx; ++ix) {
       double uu[NV], pp[NV];
@@ -1280,14 +1320,17 @@ int kzo_sim_op_implicit(kzo_sim *s, double dtin) {
         memcpy(cell(s->u, s, ix, iy, iz), uu, sizeof uu);
         memcpy(cell(s->p, s, ix, iy, iz), pp, sizeof pp);
         *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, iz) = 0;
-        s->n_iters += (uint64_t)r.iters;
-        s->n_solves += 1;
+        n_iters += (uint64_t)r.iters;
+        n_solves += 1;
       } else {
         *flagp(s, KZ_FLAG_RADIMP_FIXUP, ix, iy, iz) = -1;
-        s->n_fail += 1;
+        n_fail += 1;
       }
     }
   }
+  s->n_iters += n_iters;
+  s->n_solves += n_solves;
+  s->n_fail += n_fail;
   cell_fixup(s, KZ_FLAG_RADIMP_FIXUP);
   return KZO_OK;
 }

@@ -915,7 +915,7 @@ static int solve_4d_prim_core(const double *uu00, const kzo_radstate *state00, c
     if (failed) break;
 
     double scalevec[4] = {pp[sh], 1.0, 1.0, 1.0};
-    double invscalevec[4];
+    double invscalevec[4] = {1.0, 1.0, 1.0, 1.0};
     if (ip->scale_jacobian) {
       for (int j = 0; j < 4; ++j) {
         if (scalevec[j] == 0.0) scalevec[j] = 1.0;
