@@ -59,6 +59,12 @@ int kz_host_calc_b_from_a(const kz_grid *g, const double *gc, double *p);
 double kz_host_uniform(uint64_t seed, uint64_t cell, uint64_t var);
 int kz_host_perturb(const kz_grid *g, uint64_t seed, double amp, double gam, int rho_u_only, double *p);
 
+/* BL radius of each radial index (scalars.zig:47-50 radiusBL), r[nx] */
+int kz_host_radius_bl(const kz_grid *g, double a, double mksr0, double mksh0, double *r);
+/* BL theta and phi widths of the cells of radial shell ix, [nz][ny] each
+ * (scalars.zig:188-204 blThetaWidth / blPhiWidth) */
+int kz_host_shell_widths_bl(const kz_grid *g, double a, double mksr0, double mksh0, long ix, double *dth,
+                            double *dph);
 int kz_host_geometry_bl(const kz_grid *g, double a, double mksr0, double mksh0, long ix, long iy, long iz,
                         kz_geom *out);
 

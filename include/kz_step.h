@@ -101,6 +101,18 @@ int kz_sim_replay(kz_sim *s);
 /* 0 when everything enqueued on the sim's stream has finished, 1 while
  * work is pending (a non-blocking poll; kz_sim_sync reports errors). */
 int kz_sim_query(kz_sim *s);
+/* One scalars.dat row of the current state (koral/io/dump.zig:383-411
+ * scalarRow on one rank, over io/scalars.zig's reductions): mass, mdot
+ * through radial shell ix_h (sign-flipped), radiative and total luminosity
+ * through shell ix_l in the BL frame, scale height at ix_s, max pmag/ptot,
+ * the fixup/NaN counts and the solver counters.  dth_bl, dph_bl: the BL
+ * theta and phi widths of shell ix_l's cells, [nz][ny]
+ * (kz_host_shell_widths_bl); the shell indices come from kz_host_radius_bl
+ * (scalars.zig:56-66 radialShellIndex).  Needs kz_sim_set_bl_tables; the
+ * sums run in the reference's serial order, so the row is bitwise a serial
+ * evaluation's. */
+int kz_sim_scalars(kz_sim *s, double dt, long ix_h, long ix_l, long ix_s, const double *dth_bl, const double *dph_bl,
+                   kz_scalar_row *row);
 /* Device memory this run holds (bytes), for admission by HBM footprint. */
 int kz_sim_device_bytes(kz_sim *s, uint64_t *bytes);
 /* The hipStream_t the sim enqueues on. */

@@ -209,6 +209,15 @@ typedef struct kz_u2prad_result {
 
 /* Per-step counters (the Sim fields the self-goldens record,
  * koral/testing/selfgolden.zig:52-88, plus the solve count). */
+/* One row of scalars.dat (koral/io/dump.zig:414-436 ScalarRow): code units,
+ * mdot sign-flipped so that accretion is positive. */
+typedef struct kz_scalar_row {
+  double t, dt;
+  uint64_t nstep;
+  double mass, mdot, radlum, totallum, scaleheight, max_pmag_ptot;
+  uint64_t n_hd_fixup, n_radimp_fail, n_nan, n_floorguard;
+} kz_scalar_row;
+
 typedef struct kz_counters {
   double t, dt, own_dt, tstepdenmax, tstepdenmin;
   uint64_t nstep;

@@ -117,6 +117,13 @@ class kz_counters(C.Structure):
                                     "n_guard_recoveries")]
 
 
+class kz_scalar_row(C.Structure):
+    """One scalars.dat row (koral/io/dump.zig:414-436 ScalarRow)."""
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("nstep", C.c_uint64)] + \
+        [(n, C.c_double) for n in ("mass", "mdot", "radlum", "totallum", "scaleheight", "max_pmag_ptot")] + \
+        [(n, C.c_uint64) for n in ("n_hd_fixup", "n_radimp_fail", "n_nan", "n_floorguard")]
+
+
 IMPLICIT_RESULT_DTYPE = np.dtype([(n, np.int32) for n in ("ok", "rung", "iters", "whichprim", "whicheq",
                                                           "whichframe")])
 U2P_RESULT_DTYPE = np.dtype([(n, np.int32) for n in ("ret", "entropy_used", "fixup", "_pad")])

@@ -1888,6 +1888,8 @@ struct kz_sim {
   size_t ev_used = 0;               // events recorded since the last read
   // device bytes allocated for this run, and the captured multi-step graph
   size_t dev_bytes = 0;
+  double *scal_buf = nullptr;  // per-cell terms of the scalar diagnostics (allocated on first use)
+  double wedge = 0;            // maxz - minz of the caller's grid
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_steps = 0;
@@ -1991,6 +1993,7 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
   kz_sim *s = new kz_sim();
   s->ph = *ph;
+  s->wedge = gr->maxz - gr->minz;
   GridD &G = s->g;
   G.nx = gr->nx; G.ny = gr->ny; G.nz = gr->nz; G.ng = gr->ng;
   G.ngx = gr->nx > 1 ? gr->ng : 0;
@@ -2499,6 +2502,202 @@ int kz_sim_counters(kz_sim *s, kz_counters *c) {
   c->n_radimp_iters = cnt[1];
   c->n_radimp_solves = cnt[2];
   c->n_guard_recoveries = cnt[3];
+  return 0;
+}
+
+// ---- scalar diagnostics (koral/io/scalars.zig, io/dump.zig:358-411) -------
+// The per-cell terms of the row's sums are formed on the device with the
+// reference's expressions; the host adds them in the reference's loop order
+// (each sum is a serial fold in the reference), so the row equals a serial
+// evaluation bit for bit.
+struct ScalArgs {
+  long ix_h, ix_l, ix_s;
+  double wedge;       // maxz - minz
+  const double *dth;  // [nz][ny] BL theta widths of shell ix_l
+  const double *dph;  // [nz][ny] BL phi widths of shell ix_l
+  double *mass;       // [nd] rho dx dy dz sqrt(-g)
+  double *ratio;      // [nd] pmag / ptot
+  int32_t *diag;      // [nd] bit 0: a non-finite primitive, bit 1: HD fixup flag
+  double *shell;      // [5][nz][ny] mdot, radlum, totallum, sigma, sigma dtheta^2 terms
+};
+// hydro.zig:87-112 calcTij (T^{mu nu} of the gas and field)
+__device__ void calc_tij_full(const double *pp, const Geo &gm, double gam, double t[4][4]) {
+  double ucon[4], ucov[4], bcon[4], bcov[4], bsq;
+  ucon_ucov(pp, gm, ucon, ucov);
+  bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
+  const double rho = pp[RHO], u = pp[UU];
+  const double p = (gam - 1.0) * u;
+  const double w = rho + u + p;
+  const double eta = w + bsq;
+  const double ptot = p + 0.5 * bsq;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[i][j] = eta * ucon[i] * ucon[j] + ptot * gm.G[i][j] - bcon[i] * bcon[j];
+}
+__global__ void __launch_bounds__(128) k_scalar_terms(GridD g, Dev d, kz_physics ph, ScalArgs a) {
+  long t = TID;
+  const long nd = (long)g.nx * g.ny * g.nz;
+  if (t >= nd) return;
+  const long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  const long n = g.ncell, c = cidx(g, ix, iy, iz);
+  const long ns = (long)g.nz * g.ny, sh = iz * g.ny + iy;
+  const bool twod = g.nz == 1;
+  Geo gm;
+  geo_c(g, d, ix, iy, iz, gm);
+  double pp[NV];
+  ld(d.p, n, c, pp);
+  // totalMass (scalars.zig:79-101)
+  const double phifac = g.nz > 1 ? 2.0 * KZ_PI / a.wedge : 1.0;
+  const double dx = csize(g, ix, 0), dy = csize(g, iy, 1), dz = csize(g, iz, 2) * phifac;
+  a.mass[t] = pp[RHO] * dx * dy * dz * gm.gdet;
+  // maxPmagPtot (scalars.zig:275-313)
+  {
+    double ucon[4], ucov[4], bcon[4], bcov[4], bsq;
+    ucon_ucov(pp, gm, ucon, ucov);
+    bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
+    const double pmag = bsq / 2.0;
+    double ptot = (ph.gam - 1.0) * pp[UU];
+    const double rtt = calc_ff_rtt(pp, gm);
+    ptot += (-rtt) / 3.0;
+    a.ratio[t] = pmag / ptot;
+  }
+  // collectDiag (dump.zig:358-380)
+  int dg = 0;
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv)
+    if (!kzm_isfinite(pp[iv])) dg |= 1;
+  if (d.flags[KZ_FLAG_HD_FIXUP * n + c] != 0) dg |= 2;
+  a.diag[t] = dg;
+  // mdot (scalars.zig:108-133)
+  if (ix == a.ix_h) {
+    double v[4] = {0.0, pp[VX], pp[VY], pp[VZ]}, ucon[4];
+    velr_to_vel4(v, gm, ucon);
+    const double rhouconr = pp[RHO] * ucon[1];
+    const double dzm = twod ? 2.0 * KZ_PI : csize(g, iz, 2) * (2.0 * KZ_PI / a.wedge);
+    a.shell[0 * ns + sh] = gm.gdet * rhouconr * dy * dzm;
+  }
+  // lum, type 1, in BL (scalars.zig:141-185; frames.zig:300-310 transPallCoco)
+  if (ix == a.ix_l) {
+    Geo gbl;
+    bl_geo(d, n, c, gbl);
+    double J[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) J[k] = d.bl[(BL_J + k) * n + c];
+    double q[NV];
+#pragma unroll
+    for (int iv = 0; iv < NV; ++iv) q[iv] = pp[iv];
+    int e = trans_pmhd(q, gm, gbl, J);
+    double ur[4], vr[4] = {0.0, q[FX], q[FY], q[FZ]};
+    velr_to_vel4(vr, gm, ur);
+    tvec(J, ur, ur);
+    if (!e) e = vel4_to_velr(ur, gbl, true, ur);
+    if (e) {
+      atomicCAS(d.err, 0, e);
+      return;
+    }
+    q[FX] = ur[1];
+    q[FY] = ur[2];
+    q[FZ] = ur[3];
+    const double dthbl = a.dth[sh];
+    const double dphbl = twod ? 2.0 * KZ_PI : a.dph[sh] * (2.0 * KZ_PI / a.wedge);
+    double v[4] = {0.0, q[VX], q[VY], q[VZ]}, ucon[4];
+    velr_to_vel4(v, gbl, ucon);
+    const double rhour = q[RHO] * ucon[1];
+    double tu[4][4], tl[4][4];
+    calc_tij_full(q, gbl, ph.gam, tu);
+    lower_second(tu, gbl, tl);
+    const double trt = tl[1][0];
+    double ru[4][4], rl[4][4];
+    calc_rij(q, gbl, ru);
+    lower_second(ru, gbl, rl);
+    double rrt = -rl[1][0];
+    if (rrt < 0) rrt = 0;
+    a.shell[1 * ns + sh] = gbl.gdet * rrt * dthbl * dphbl;
+    a.shell[2 * ns + sh] = gbl.gdet * (rhour + trt + rrt) * dthbl * dphbl;
+  }
+  // scaleHeightPartsAtIx (dynamo.zig:239-262)
+  if (ix == a.ix_s) {
+    const double th = d.bl[BL_TH * n + c];
+    const double dthe = KZ_PI / 2.0 - th;
+    a.shell[3 * ns + sh] = pp[RHO] * gm.gdet;
+    a.shell[4 * ns + sh] = pp[RHO] * gm.gdet * dthe * dthe;
+  }
+}
+
+int kz_sim_scalars(kz_sim *s, double dt, long ix_h, long ix_l, long ix_s, const double *dth_bl, const double *dph_bl,
+                   kz_scalar_row *row) {
+  const GridD &g = s->g;
+  if (!s->has_bl || !row || !dth_bl || !dph_bl) return KZ_E_INVALID_CONFIG;
+  if (ix_h < 0 || ix_h >= g.nx || ix_l < 0 || ix_l >= g.nx || ix_s < 0 || ix_s >= g.nx) return KZ_E_INVALID_CONFIG;
+  const long nd = (long)g.nx * g.ny * g.nz, ns = (long)g.nz * g.ny;
+  // layout: mass[nd] ratio[nd] shell[5 ns] dth[ns] dph[ns] diag[nd] (int32)
+  const size_t nbytes = (size_t)(2 * nd + 7 * ns) * sizeof(double) + (size_t)nd * sizeof(int32_t);
+  if (!s->scal_buf) {
+    void *p = nullptr;
+    if (int e = dalloc(s, &p, nbytes)) return e;
+    s->scal_buf = (double *)p;
+  }
+  ScalArgs a;
+  a.ix_h = ix_h;
+  a.ix_l = ix_l;
+  a.ix_s = ix_s;
+  a.wedge = s->wedge;
+  a.mass = s->scal_buf;
+  a.ratio = a.mass + nd;
+  a.shell = a.ratio + nd;
+  double *ddth = a.shell + 5 * ns, *ddph = ddth + ns;
+  a.dth = ddth;
+  a.dph = ddph;
+  a.diag = (int32_t *)(ddph + ns);
+  HIPCHK(hipMemcpyAsync(ddth, dth_bl, ns * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(ddph, dph_bl, ns * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  k_scalar_terms<<<nblk(nd, 128), 128, 0, s->stream>>>(g, s->d, s->ph, a);
+  if (int e = check_err(s)) return e;
+  std::vector<double> mass(nd), ratio(nd), shell(5 * ns);
+  std::vector<int32_t> diag(nd);
+  HIPCHK(hipMemcpyAsync(mass.data(), a.mass, nd * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(ratio.data(), a.ratio, nd * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(shell.data(), a.shell, 5 * ns * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(diag.data(), a.diag, nd * sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // the reference's serial folds: mass and the maximum over (iz, iy, ix),
+  // the shell sums over (iz, iy), the scale-height parts over (iy, iz)
+  double m = 0, maxb = 0;
+  uint64_t nhd = 0, nnan = 0;
+  for (long t = 0; t < nd; ++t) {
+    m += mass[t];
+    if (ratio[t] > maxb) maxb = ratio[t];
+    if (diag[t] & 1) nnan += 1;
+    if (diag[t] & 2) nhd += 1;
+  }
+  double md = 0, radlum = 0, totallum = 0;
+  for (long sh = 0; sh < ns; ++sh) {
+    md += shell[sh];
+    radlum += shell[ns + sh];
+    totallum += shell[2 * ns + sh];
+  }
+  double sig = 0, sth = 0;
+  for (long iy = 0; iy < g.ny; ++iy)
+    for (long iz = 0; iz < g.nz; ++iz) {
+      sig += shell[3 * ns + iz * g.ny + iy];
+      sth += shell[4 * ns + iz * g.ny + iy];
+    }
+  kz_counters c;
+  if (int e = kz_sim_counters(s, &c)) return e;
+  row->t = c.t;
+  row->dt = dt;
+  row->nstep = c.nstep;
+  row->mass = m;
+  row->mdot = -md;
+  row->radlum = radlum;
+  row->totallum = totallum;
+  row->scaleheight = ix_s > 0 ? std::sqrt(sth / sig) : sth;
+  row->max_pmag_ptot = maxb;
+  row->n_hd_fixup = nhd;
+  row->n_radimp_fail = c.n_radimp_failures;
+  row->n_nan = nnan;
+  row->n_floorguard = c.n_guard_recoveries;
   return 0;
 }
 

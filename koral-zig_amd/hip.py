@@ -44,6 +44,8 @@ _SIGS = {
     "kz_sim_replay": (C.c_int, [C.c_void_p]),
     "kz_sim_query": (C.c_int, [C.c_void_p]),
     "kz_sim_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_sim_scalars": (C.c_int, [C.c_void_p, C.c_double, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_void_p,
+                                 C.POINTER(T.kz_scalar_row)]),
     "kz_sim_counters": (C.c_int, [C.c_void_p, C.POINTER(T.kz_counters)]),
     "kz_sim_time_kernel": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_kernel_time": (C.c_int, [C.c_void_p, D, C.POINTER(C.c_uint64)]),
@@ -248,6 +250,15 @@ class Sim:
         if rc < 0:
             _chk(rc, "kz_sim_query")
         return rc == 1
+
+    def scalars(self, dt, ix_h, ix_l, ix_s, dth_bl, dph_bl):
+        """One scalars.dat row of the current state (kz_sim_scalars) as a dict
+        in the row's column order; koral_zig_amd.scalars drives it."""
+        dth, dph = _f64(dth_bl), _f64(dph_bl)
+        r = T.kz_scalar_row()
+        _chk(lib().kz_sim_scalars(self.h, float(dt), int(ix_h), int(ix_l), int(ix_s), dth.ctypes.data,
+                                  dph.ctypes.data, C.byref(r)), "kz_sim_scalars")
+        return {n: getattr(r, n) for n, _ in T.kz_scalar_row._fields_}
 
     def device_bytes(self):
         n = C.c_uint64(0)

@@ -46,6 +46,8 @@ def lib():
         L.kz_host_geometry_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_long, C.c_long, C.c_long,
                                           C.c_void_p]
         L.kz_host_bl_tables.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
+        L.kz_host_radius_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
+        L.kz_host_shell_widths_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_long, D, D]
         _lib = L
     return _lib
 
@@ -172,3 +174,19 @@ def bl_tables(grid, mp=MP_PUFFY):
     out = np.zeros(nc * BL_REC)
     L.kz_host_bl_tables(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"], T.ptr(out))
     return out
+
+
+def radius_bl(grid, mp=MP_PUFFY):
+    """BL radius of each radial index (scalars.zig:47-50 radiusBL)."""
+    r = np.zeros(grid.nx)
+    lib().kz_host_radius_bl(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"], T.ptr(r))
+    return r
+
+
+def shell_widths_bl(grid, ix, mp=MP_PUFFY):
+    """BL theta and phi widths of the cells of radial shell ix, [nz][ny] each
+    (scalars.zig:188-204)."""
+    dth = np.zeros((grid.nz, grid.ny))
+    dph = np.zeros((grid.nz, grid.ny))
+    lib().kz_host_shell_widths_bl(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"], int(ix), T.ptr(dth), T.ptr(dph))
+    return dth, dph

@@ -1211,6 +1211,41 @@ int kz_host_bl_tables(const kz_grid *gr, double a, double mksr0, double mksh0, d
   return 0;
 }
 
+// scalars.zig:47-50 radiusBL: the BL radius of each radial index, at the
+// centre of the first theta and phi cells
+int kz_host_radius_bl(const kz_grid *gr, double a, double mksr0, double mksh0, double *r) {
+  Grid G(*gr);
+  MP mp{a, mksr0, mksh0};
+  for (long ix = 0; ix < gr->nx; ++ix) {
+    double x[4] = {0.0, G.xc(ix), G.yc(0), G.zc(0)}, o[4];
+    coco_mks2_to_bl(x, mp, o);
+    r[ix] = o[1];
+  }
+  return 0;
+}
+
+// scalars.zig:188-204 blThetaWidth / blPhiWidth at radial index ix, for every
+// (iz, iy) of the shell: |theta_BL| across the two y faces at the x centre,
+// |phi_BL| across the two z faces at the cell centre; row-major [nz][ny]
+int kz_host_shell_widths_bl(const kz_grid *gr, double a, double mksr0, double mksh0, long ix, double *dth,
+                            double *dph) {
+  Grid G(*gr);
+  MP mp{a, mksr0, mksh0};
+  for (long iz = 0; iz < gr->nz; ++iz)
+    for (long iy = 0; iy < gr->ny; ++iy) {
+      double xa[4] = {0.0, G.xc(ix), G.yl(iy), G.zc(iz)}, xb[4] = {0.0, G.xc(ix), G.yl(iy + 1), G.zc(iz)};
+      double oa[4], ob[4];
+      coco_mks2_to_bl(xa, mp, oa);
+      coco_mks2_to_bl(xb, mp, ob);
+      dth[iz * gr->ny + iy] = std::fabs(ob[2] - oa[2]);
+      double za[4] = {0.0, G.xc(ix), G.yc(iy), G.zl(iz)}, zb[4] = {0.0, G.xc(ix), G.yc(iy), G.zl(iz + 1)};
+      coco_mks2_to_bl(za, mp, oa);
+      coco_mks2_to_bl(zb, mp, ob);
+      dph[iz * gr->ny + iy] = std::fabs(ob[3] - oa[3]);
+    }
+  return 0;
+}
+
 int kz_host_geometry_bl(const kz_grid *gr, double a, double mksr0, double mksh0, long ix, long iy, long iz,
                         kz_geom *out) {
   Grid G(*gr);

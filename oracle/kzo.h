@@ -76,6 +76,11 @@ enum { WP_RAD = 1, WP_MHD = 2 };
 enum { WE_ENERGY = 0, WE_ENTROPY = 1 };
 enum { WF_LAB = 0, WF_FF = 1 };
 
+/* the reductions of one scalars.dat row (io/scalars.zig), see kzo_sim.c */
+typedef struct kzo_sim kzo_sim;
+int kzo_sim_scalars(kzo_sim *s, long ix_h, long ix_l, long ix_s, const double *dth, const double *dph, double out[6],
+                    uint64_t cnt[2]);
+
 /* threads of the oracle sim's cell-parallel passes (results independent of it) */
 void kzo_set_threads(int n);
 int kzo_get_threads(void);

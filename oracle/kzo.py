@@ -89,6 +89,7 @@ def lib():
         L.kzo_r_horizon_bl.restype = C.c_double
         L.kzo_r_isco_bl.argtypes = [C.c_double]
         L.kzo_r_isco_bl.restype = C.c_double
+        L.kzo_sim_scalars.argtypes = [vp, C.c_long, C.c_long, C.c_long, vp, vp, vp, vp]
         L.kzo_sim_counters.argtypes = [vp, C.POINTER(T.kz_counters)]
         L.kzo_sim_counters.restype = None
         L.kzo_sim_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_double]
@@ -385,6 +386,20 @@ class Sim:
         s = np.zeros(self.shape + (20,))
         lib().kzo_sim_get_scal(self.h, T.ptr(s))
         return s
+
+    def scalars(self, dt, ix_h, ix_l, ix_s, dth_bl, dph_bl):
+        """One scalars.dat row (dump.zig:383-411 scalarRow on one rank)."""
+        dth = np.ascontiguousarray(dth_bl, np.float64)
+        dph = np.ascontiguousarray(dph_bl, np.float64)
+        out = np.zeros(6)
+        cnt = np.zeros(2, np.uint64)
+        self._chk(lib().kzo_sim_scalars(self.h, int(ix_h), int(ix_l), int(ix_s), dth.ctypes.data, dph.ctypes.data,
+                                        out.ctypes.data, cnt.ctypes.data), "kzo_sim_scalars")
+        c = self.counters()
+        return {"t": c["t"], "dt": float(dt), "nstep": c["nstep"], "mass": out[0], "mdot": -out[1],
+                "radlum": out[2], "totallum": out[3], "scaleheight": out[4], "max_pmag_ptot": out[5],
+                "n_hd_fixup": int(cnt[0]), "n_radimp_fail": c["n_radimp_failures"], "n_nan": int(cnt[1]),
+                "n_floorguard": c["n_guard_recoveries"]}
 
     def counters(self):
         c = T.kz_counters()

@@ -1072,6 +1072,136 @@ static int mimic_dynamo(kzo_sim *s, double dt) {
   return KZO_OK;
 }
 
+/* ---- scalar diagnostics (io/scalars.zig, io/dump.zig:358-411) ---- */
+/* frames.zig:300-310 transPallCoco to BL with the sidecar's point Jacobian:
+ * transPmhdCocoJ, then transPradCoco (VELR -> VEL4 recomputing u^t, the
+ * Jacobian, VEL4 -> VELR trusting u^t) */
+static int trans_pall_bl(double *pp, const kzo_geo *gm, const kzo_geo *gbl, const double *J) {
+  int e = trans_pmhd(pp, gm, gbl, J);
+  if (e) return e;
+  double ucon[4] = {0.0, pp[FX], pp[FY], pp[FZ]};
+  e = kzo_convert(ucon, VELR, VEL4, gm, 0, ucon);
+  if (e) return e;
+  tvec(J, ucon, ucon);
+  e = kzo_convert(ucon, VEL4, VELR, gbl, 1, ucon);
+  if (e) return e;
+  pp[FX] = ucon[1]; pp[FY] = ucon[2]; pp[FZ] = ucon[3];
+  return KZO_OK;
+}
+
+/* The reductions of one scalars.dat row on one rank (scalars.zig:79-313
+ * totalMass, mdot, lum, scaleHeightPartsAtIx + scaleHeightFinalize,
+ * maxPmagPtot; dump.zig:358-380 collectDiag), in the reference's loop order.
+ * ix_h, ix_l, ix_s: radial indices of the mass-flux, luminosity and
+ * scale-height shells; dth, dph: BL theta / phi widths of shell ix_l's cells,
+ * [nz][ny].  out: mass, mdot (net, not sign-flipped), radlum, totallum,
+ * scaleheight, max pmag/ptot; cnt: cells with an HD fixup flag, cells with a
+ * non-finite primitive.  Needs the BL sidecar. */
+int kzo_sim_scalars(kzo_sim *s, long ix_h, long ix_l, long ix_s, const double *dth, const double *dph, double out[6],
+                    uint64_t cnt[2]) {
+  const long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  const int twod = nz == 1;
+  const double wedge = s->grid.maxz - s->grid.minz;
+  if (!s->bl) return KZ_E_INVALID_CONFIG;
+  /* totalMass (scalars.zig:79-101) */
+  const double phifac = nz > 1 ? 2.0 * KZO_PI / wedge : 1.0;
+  double mass = 0;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        kzo_geo g;
+        geo_c(s, ix, iy, iz, &g);
+        double dx = csize(s, ix, 0), dy = csize(s, iy, 1), dz = csize(s, iz, 2) * phifac;
+        mass += cell(s->p, s, ix, iy, iz)[RHO] * dx * dy * dz * g.gdet;
+      }
+  /* mdot (scalars.zig:108-133) */
+  double md = 0;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy) {
+      const double *pp = cell(s->p, s, ix_h, iy, iz);
+      kzo_geo g;
+      geo_c(s, ix_h, iy, iz, &g);
+      double v[4] = {0.0, pp[VX], pp[VY], pp[VZ]}, ucon[4];
+      int e = kzo_convert(v, VELR, VEL4, &g, 0, ucon);
+      if (e) return e;
+      double rhouconr = pp[RHO] * ucon[1];
+      double dy = csize(s, iy, 1);
+      double dz = twod ? 2.0 * KZO_PI : csize(s, iz, 2) * (2.0 * KZO_PI / wedge);
+      md += g.gdet * rhouconr * dy * dz;
+    }
+  /* lum, type 1 (scalars.zig:141-185) */
+  double radlum = 0, totallum = 0;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy) {
+      double pp[NV];
+      memcpy(pp, cell(s->p, s, ix_l, iy, iz), sizeof pp);
+      kzo_geo g, gbl;
+      geo_c(s, ix_l, iy, iz, &g);
+      bl_geo(s, ix_l, iy, iz, &gbl);
+      int e = trans_pall_bl(pp, &g, &gbl, bl_rec(s, ix_l, iy, iz) + 36);
+      if (e) return e;
+      double dthbl = dth[iz * ny + iy];
+      double dphbl = twod ? 2.0 * KZO_PI : dph[iz * ny + iy] * (2.0 * KZO_PI / wedge);
+      double v[4] = {0.0, pp[VX], pp[VY], pp[VZ]}, ucon[4];
+      e = kzo_convert(v, VELR, VEL4, &gbl, 0, ucon);
+      if (e) return e;
+      double rhour = pp[RHO] * ucon[1];
+      double tu[4][4], tl[4][4];
+      e = kzo_calc_tij(pp, &gbl, s->ph.gam, tu);
+      if (e) return e;
+      kzo_lower_second(tu, &gbl, tl);
+      double trt = tl[1][0];
+      double ru[4][4], rl[4][4];
+      kzo_calc_rij(pp, &gbl, ru);
+      kzo_lower_second(ru, &gbl, rl);
+      double rrt = -rl[1][0];
+      if (rrt < 0) rrt = 0;
+      radlum += gbl.gdet * rrt * dthbl * dphbl;
+      totallum += gbl.gdet * (rhour + trt + rrt) * dthbl * dphbl;
+    }
+  /* scaleHeightPartsAtIx + scaleHeightFinalize (dynamo.zig:239-269) */
+  double sig = 0, sth = 0;
+  for (long iy = 0; iy < ny; ++iy)
+    for (long iz = 0; iz < nz; ++iz) {
+      kzo_geo g;
+      geo_c(s, ix_s, iy, iz, &g);
+      double gd = g.gdet, th = bl_rec(s, ix_s, iy, iz)[35];
+      double rho = cell(s->p, s, ix_s, iy, iz)[RHO];
+      double dthe = KZO_PI / 2.0 - th;
+      sig += rho * gd;
+      sth += rho * gd * dthe * dthe;
+    }
+  double hr = ix_s > 0 ? kzm_sqrt(sth / sig) : sth;
+  /* maxPmagPtot (scalars.zig:275-313) and collectDiag */
+  double maxb = 0;
+  uint64_t nhd = 0, nnan = 0;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        const double *pp = cell(s->p, s, ix, iy, iz);
+        kzo_geo g;
+        geo_c(s, ix, iy, iz, &g);
+        double ucon[4], ucov[4];
+        kzo_ucon_ucov_from_prims(pp, &g, ucon, ucov);
+        double B[3] = {pp[B1], pp[B2], pp[B3]}, bcon[4], bcov[4], bsq;
+        kzo_bcon_bcov_bsq(B, ucon, ucov, &g, bcon, bcov, &bsq);
+        double pmag = bsq / 2.0;
+        double ptot = (s->ph.gam - 1.0) * pp[UU];
+        double rtt = kzo_calc_ff_rtt(pp, &g, 0);
+        ptot += (-rtt) / 3.0;
+        if (pmag / ptot > maxb) maxb = pmag / ptot;
+        for (int iv = 0; iv < NV; ++iv)
+          if (!kzm_isfinite(pp[iv])) {
+            nnan += 1;
+            break;
+          }
+        if (*flagp(s, KZ_FLAG_HD_FIXUP, ix, iy, iz) != 0) nhd += 1;
+      }
+  out[0] = mass; out[1] = md; out[2] = radlum; out[3] = totallum; out[4] = hr; out[5] = maxb;
+  cnt[0] = nhd; cnt[1] = nnan;
+  return KZO_OK;
+}
+
 /* dynamo.zig:462-469 applyDynamo */
 static int apply_dynamo(kzo_sim *s, double dt) {
   calc_scale_height(s);
