@@ -27,8 +27,9 @@ typedef struct kz_sim kz_sim;
 /* Sim.init (koral/sim.zig:309-500) with the MetricCache blocks computed by
  * the caller (kz_host_metric_tables) and the PUFFY outer-boundary table
  * (kz_host_bc_xhi_tables). Returns NULL and sets *err on failure:
- * KZ_E_UNSUPPORTED for nz != 1 or an option outside this build
- * (reduce-order, synchrotron bridge, ZAMO floors), KZ_E_INVALID_CONFIG for
+ * KZ_E_UNSUPPORTED for a grid with a collapsed x or y dimension, fewer than
+ * 3 ghosts, or an option outside this build (reduce-order, synchrotron
+ * bridge, ZAMO floors), KZ_E_INVALID_CONFIG for
  * sim.zig:319-363's preconditions (including radiative viscosity without
  * opacities).  With radviscosity or dynamo set, the BL sidecar must be
  * loaded with kz_sim_set_bl_tables before the first step. */
@@ -69,7 +70,8 @@ int kz_do_correct(kz_sim *s);
 /* The MetricCache BL sidecar (koral/metric/precompute.zig:148,379,457-470:
  * blGeom + jacMy2Bl) as kz_host_bl_tables builds it: KZ_BL_REC doubles per
  * padded cell (BL g[16], G[16], alpha, gdet, r, theta, J_mks2->bl[16]).
- * KZ_E_INVALID_CONFIG unless radviscosity or dynamo is on. */
+ * Radiative viscosity, the dynamo and kz_sim_scalars read it; a run with
+ * neither of the first two allocates its copy on the first call. */
 int kz_sim_set_bl_tables(kz_sim *s, const double *bl);
 /* calcRijViscTotal (koral/sim/rijvisc.zig:266-311): the viscous R^i_j on the
  * domain and its one-cell ring (corners 0) for the step size global_dt;

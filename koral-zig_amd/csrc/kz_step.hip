@@ -2318,8 +2318,12 @@ int kz_do_correct(kz_sim *s) {
 }
 
 int kz_sim_set_bl_tables(kz_sim *s, const double *bl) {
-  if (!s->d.bl) return KZ_E_INVALID_CONFIG;
+  if (!bl) return KZ_E_INVALID_CONFIG;
   const long n = s->g.ncell;
+  // runs without radiative viscosity or the dynamo allocate the sidecar here
+  // (the scalar diagnostics read it)
+  if (!s->d.bl)
+    if (int e = dalloc(s, (void **)&s->d.bl, (size_t)n * BL_NC * sizeof(double))) return e;
   std::vector<double> planes((size_t)n * BL_NC);
   for (long c = 0; c < n; ++c)
     for (int k = 0; k < BL_NC; ++k) planes[(size_t)k * n + c] = bl[(size_t)c * BL_NC + k];
