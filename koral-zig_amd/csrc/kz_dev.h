@@ -375,8 +375,28 @@ KZD Opac calc_opacities(const kz_opac_params &c, double rho, double te, double t
       double iab = bmagcgs * bmagcgs / den;
       double te5 = te * te * te * te * te;
       krsyn = c.kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * iab) * rho;
-      double bmb = bmagcgs > 0.0 ? kzm_pow(bmagcgs, -0.463) : 0.0;
-      double iabr = 0.13 * kzm_pow(bmagcgs, 0.69) / (kzm_pow(zbr, 1.69) * kzm_exp(1.6 * kzm_pow(zbr, 0.463) * bmb));
+      // two powers of each base: one logarithm per base (kzm_pow_from_log
+      // returns kzm_pow's bits for a finite positive base)
+      double bmb, pb69, pz169, pz463;
+      if (bmagcgs > 0.0 && kzm_isfinite(bmagcgs)) {
+        double lh, ll;
+        kzm_log_dd(bmagcgs, &lh, &ll);
+        bmb = kzm_pow_from_log(lh, ll, -0.463);
+        pb69 = kzm_pow_from_log(lh, ll, 0.69);
+      } else {
+        bmb = bmagcgs > 0.0 ? kzm_pow(bmagcgs, -0.463) : 0.0;
+        pb69 = kzm_pow(bmagcgs, 0.69);
+      }
+      if (zbr > 0.0 && kzm_isfinite(zbr)) {
+        double lh, ll;
+        kzm_log_dd(zbr, &lh, &ll);
+        pz169 = kzm_pow_from_log(lh, ll, 1.69);
+        pz463 = kzm_pow_from_log(lh, ll, 0.463);
+      } else {
+        pz169 = kzm_pow(zbr, 1.69);
+        pz463 = kzm_pow(zbr, 0.463);
+      }
+      double iabr = 0.13 * pb69 / (pz169 * kzm_exp(1.6 * pz463 * bmb));
       krsynross = c.kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * iabr) * rho;
     }
     double terel = te * c.k_over_mecsq;

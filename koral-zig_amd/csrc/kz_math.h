@@ -238,6 +238,21 @@ KZM_QUAL double kzm_pow(double x, double y) {
   return sign * kzm_exp_dd(th, tl);
 }
 
+/* kzm_pow(x, y) for finite x > 0 and finite y, from log(x) as kzm_log_dd
+ * returns it: the operations kzm_pow performs after its special cases (and
+ * 1.0 for y == 0 and for x == 1, where log_dd gives exactly zero), so a
+ * caller taking several powers of one base can form the logarithm once and
+ * get kzm_pow's bits for each. */
+KZM_QUAL double kzm_pow_from_log(double lh, double ll, double y) {
+  if (y == 0.0) return 1.0;
+  double zh, zl;
+  kzm_two_prod(y, lh, &zh, &zl);
+  zl += y * ll;
+  double th, tl;
+  kzm_fast_two_sum(zh, zl, &th, &tl);
+  return kzm_exp_dd(th, tl);
+}
+
 /* ---- cbrt ----------------------------------------------------------------- */
 KZM_QUAL double kzm_cbrt(double x) {
   if (x == 0.0 || x != x || !kzm_isfinite(x)) return x;

@@ -281,7 +281,7 @@ def invert4f(a16):
     return out
 
 
-MATH_FN = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "log1p": 4, "sqrt": 5, "div": 6}
+MATH_FN = {"exp": 0, "log": 1, "pow": 2, "cbrt": 3, "log1p": 4, "sqrt": 5, "div": 6, "pow_from_log": 7}
 
 
 def math_fn(name, x, y=None):

@@ -182,7 +182,8 @@ int kzo_invert4f_batch(int64_t n, const float *a16, float *inv16) {
   return 0;
 }
 
-/* fn: 0 exp 1 log 2 pow 3 cbrt 4 log1p 5 sqrt 6 div */
+/* fn: 0 exp 1 log 2 pow 3 cbrt 4 log1p 5 sqrt 6 div 7 pow through
+ * kzm_pow_from_log (finite x > 0 only) */
 int kzo_math_batch(int64_t n, int32_t fn, const double *x, const double *y, double *out) {
   for (int64_t i = 0; i < n; ++i) {
     double a = x[i], b = y ? y[i] : 0.0, r;
@@ -193,6 +194,13 @@ int kzo_math_batch(int64_t n, int32_t fn, const double *x, const double *y, doub
       case 3: r = kzm_cbrt(a); break;
       case 4: r = kzm_log1p(a); break;
       case 5: r = kzm_sqrt(a); break;
+      case 7: {
+        double lh, ll;
+        kzm_log_dd(a, &lh, &ll);
+        r = kzm_pow_from_log(lh, ll, b);
+        break;
+      }
+      case 6:
       default: r = a / b; break;
     }
     out[i] = r;
