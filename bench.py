@@ -52,9 +52,9 @@ IMPLICIT_FLOPS_PER_ITER = 10609.1
 # The independent-cell batch spends most of its operations in the later
 # rungs and in failing ladders, which the fit above does not describe:
 # `python oracle/flops/count.py c2` over 200 000 cells of the same generator
-# counts 311 175.3 operations per cell (first-rung solves 91 938.7, later
-# rungs 720 573.7, failed cells 2 666 322.8).
-BATCH_FLOPS_PER_CELL = 311175.3
+# counts 152 267.7 operations per cell (first-rung solves 63 516.7, later
+# rungs 768 799.1, failed cells 2 970 648.7).
+BATCH_FLOPS_PER_CELL = 152267.7
 # HBM bytes per launch of a kernel class at 384x360, from rocprofv3 runs of
 # FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
 # counter reads for a copy kernel of known size in the same run (k_soa2aos:

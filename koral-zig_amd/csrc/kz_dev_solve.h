@@ -771,154 +771,177 @@ KZD double fd_del(int j, double sign, const double *b, double eps, const Geo &gm
   return bj >= 0.0 ? sign * veleps : -sign * veleps;
 }
 
-// implicit.zig:642-922 solve4dPrimCore (scalar Jacobian); returns ok
+// implicit.zig:642-922 solve4dPrimCore (scalar Jacobian), as a start and a
+// one-iteration step so that a persistent kernel can interleave cells at
+// different iterations: newton_step runs one pass of the reference's
+// iteration loop and returns 0 to go on, 1 when the loop ends solved (n.pp
+// and n.iter hold the result) and -1 when it ends failed.
+struct NewtonState {
+  double uu[NV], pp[NV];
+  int iter, fret;
+};
 template <int WP>
-__device__ static __forceinline__ bool solve_4d_prim_core(const double *uu00, const LagOpac &lag, const State0 &s0, const double *pp0in, double dt,
-                            const ImpCtx &cx, int we, int wf, double *pp_out, int *iters_out) {
-  const kz_implicit_params &ip = *cx.ip;
-  const Geo &gm = *cx.gm;
-  double uu[NV], pp[NV], ppp[NV];
+KZD void newton_start(NewtonState &n, const double *uu00, const double *pp0in, const ImpCtx &cx) {
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    uu[i] = uu00[i];
-    pp[i] = pp0in[i];
+    n.uu[i] = uu00[i];
+    n.pp[i] = pp0in[i];
   }
+  n.fret = apply_constraints<WP>(n.pp, n.uu, uu00, cx);
+  n.iter = 0;
+}
+template <int WP>
+KZD int newton_step(NewtonState &n, const double *uu00, const LagOpac &lag, const State0 &s0, double dt,
+                    const ImpCtx &cx, int we, int wf) {
+  const kz_implicit_params &ip = *cx.ip;
+  const Geo &gm = *cx.gm;
+  double *uu = n.uu, *pp = n.pp;
+  double ppp[NV];
   constexpr int sh = (WP == WP_MHD) ? UU : EE;
   const double conv = (we == WE_ENTROPY) ? ip.entrconv : ip.conv;
-  int fret = apply_constraints<WP>(pp, uu, uu00, cx);
   double f1[4], f2[4], jac[4][4];
-  int iter = 0;
   bool failed = false;
-  for (;;) {
-    iter += 1;
+  int &fret = n.fret;
+  const int iter = ++n.iter;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) ppp[i] = pp[i];
-    if (fret < -1) return false;
-    double errbase = eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f1);
-    if (errbase < conv) break;
+  for (int i = 0; i < NV; ++i) ppp[i] = pp[i];
+  if (fret < -1) return -1;
+  double errbase = eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f1);
+  if (errbase < conv) return iter > ip.maxiter ? -1 : 1;
 #pragma unroll 1
-    for (int j = 0; j < 4; ++j) {
-      double sign = -1.0;
-      for (;;) {
-        double del = fd_del(j, sign, ppp + sh, ip.eps, gm);
-        set4(pp + sh, j, get4(ppp + sh, j) + del);
-        fret = apply_constraints<WP>(pp, uu, uu00, cx);
-        if (fret < -1) {
-          if (sign > 0.0) {
-            failed = true;
-            break;
-          }
-          set4(pp + sh, j, get4(ppp + sh, j));
-          sign *= -1.0;
-          continue;
+  for (int j = 0; j < 4; ++j) {
+    double sign = -1.0;
+    for (;;) {
+      double del = fd_del(j, sign, ppp + sh, ip.eps, gm);
+      set4(pp + sh, j, get4(ppp + sh, j) + del);
+      fret = apply_constraints<WP>(pp, uu, uu00, cx);
+      if (fret < -1) {
+        if (sign > 0.0) {
+          failed = true;
+          break;
         }
-        eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f2);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) set4(jac[i], j, (f2[i] - f1[i]) / del);
         set4(pp + sh, j, get4(ppp + sh, j));
-        break;
+        sign *= -1.0;
+        continue;
       }
-      if (failed) break;
+      eval_residual<WP>(uu, pp, lag, uu00, s0, dt, cx, we, wf, f2);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) set4(jac[i], j, (f2[i] - f1[i]) / del);
+      set4(pp + sh, j, get4(ppp + sh, j));
+      break;
     }
     if (failed) break;
-    double sv[4] = {pp[sh], 1.0, 1.0, 1.0}, isv[4];
+  }
+  if (failed) return -1;
+  double sv[4] = {pp[sh], 1.0, 1.0, 1.0}, isv[4];
+  if (ip.scale_jacobian) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (sv[j] == 0.0) sv[j] = 1.0;
+      isv[j] = 1.0 / sv[j];
+    }
+  }
+  double ij[4][4];
+  if (ip.jacobian_fp32) {
+    // mixed precision: FP64 quotients rounded to FP32, scaled and
+    // factored in FP32, the inverse applied in FP64
+    float jf[4][4], ijf[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) jf[i][j] = (float)jac[i][j];
     if (ip.scale_jacobian) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (sv[j] == 0.0) sv[j] = 1.0;
-        isv[j] = 1.0 / sv[j];
-      }
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (i != j) jf[i][j] = jf[i][j] * ((float)sv[j] * (float)isv[i]);
     }
-    double ij[4][4];
-    if (ip.jacobian_fp32) {
-      // mixed precision: FP64 quotients rounded to FP32, scaled and
-      // factored in FP32, the inverse applied in FP64
-      float jf[4][4], ijf[4][4];
+    invert4_t<float>(jf, ijf);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ij[i][j] = (double)ijf[i][j];
+  } else {
+    if (ip.scale_jacobian) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) jf[i][j] = (float)jac[i][j];
-      if (ip.scale_jacobian) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (i != j) jf[i][j] = jf[i][j] * ((float)sv[j] * (float)isv[i]);
-      }
-      invert4_t<float>(jf, ijf);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ij[i][j] = (double)ijf[i][j];
-    } else {
-      if (ip.scale_jacobian) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (i != j) jac[i][j] = jac[i][j] * (sv[j] * isv[i]);
-      }
-      invert4(jac, ij);
+        for (int j = 0; j < 4; ++j)
+          if (i != j) jac[i][j] = jac[i][j] * (sv[j] * isv[i]);
     }
-    double xiapp = 1.0;
-    double xxx[4];
-    for (;;) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) xxx[k] = ppp[k + sh];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (ip.scale_jacobian) xxx[i] -= xiapp * (ij[i][j] * sv[i] * isv[j]) * f1[j];
-          else xxx[i] -= xiapp * ij[i][j] * f1[j];
-        }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) pp[k + sh] = xxx[k];
-      fret = apply_constraints<WP>(pp, uu, uu00, cx);
-      bool ok = false;
-      double edenmin = ppp[sh] / ip.max_en_change_down;
-      double edenmax = ppp[sh] * ip.max_en_change_up;
-      if (xxx[0] > edenmin && xxx[0] < edenmax && fret >= -1) ok = true;
-      double tgo = t_from_urho(*cx.op, ppp[UU], ppp[RHO], cx.gam);
-      double tgn = t_from_urho(*cx.op, pp[UU], pp[RHO], cx.gam);
-      if (ok && (tgn > tgo * ip.max_tgas_change || tgn < tgo / ip.max_tgas_change)) ok = false;
-      double trn = lte_t_from_e(*cx.op, pp[EE]);
-      double tro = lte_t_from_e(*cx.op, ppp[EE]);
-      if (ok && (trn > tro * ip.max_trad_change || trn < tro / ip.max_trad_change)) ok = false;
-      if (ok) break;
-      if (xxx[0] <= 0.0) {
-        xiapp *= ppp[sh] / (ppp[sh] + kzm_fabs(xxx[0]));
-        xiapp *= 1.0e-1;
-      } else {
-        xiapp /= ip.damping_factor;
-      }
-      if (xiapp < ip.max_damping) {
-        failed = true;
-        break;
-      }
-    }
-    if (!failed) {
-      double f3[4];
-      f3[0] = kzm_fabs((pp[sh] - ppp[sh]) / ppp[sh]);
-#pragma unroll
-      for (int i = 1; i < 4; ++i) {
-        double d = pp[i + sh] - ppp[i + sh];
-        f3[i] = kzm_fabs(d / kzm_fmax(ip.eps, kzm_fabs(ppp[i + sh])));
-      }
-      double cr = (we == WE_ENTROPY) ? ip.convrelentr : ip.convrel;
-      double cre = (we == WE_ENTROPY) ? ip.convrelentrerr : ip.convrelerr;
-      bool cc = true;
-#pragma unroll
-      for (int ib = 0; ib < 4; ++ib)
-        if (f3[ib] > cr) cc = false;
-      if (cc || errbase <= cre) break;
-    }
-    if (iter > ip.maxiter || failed) break;
+    invert4(jac, ij);
   }
-  if (iter > ip.maxiter || failed) return false;
+  double xiapp = 1.0;
+  double xxx[4];
+  for (;;) {
 #pragma unroll
-  for (int i = 0; i < NV; ++i) pp_out[i] = pp[i];
-  *iters_out = iter;
+    for (int k = 0; k < 4; ++k) xxx[k] = ppp[k + sh];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (ip.scale_jacobian) xxx[i] -= xiapp * (ij[i][j] * sv[i] * isv[j]) * f1[j];
+        else xxx[i] -= xiapp * ij[i][j] * f1[j];
+      }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pp[k + sh] = xxx[k];
+    fret = apply_constraints<WP>(pp, uu, uu00, cx);
+    bool ok = false;
+    double edenmin = ppp[sh] / ip.max_en_change_down;
+    double edenmax = ppp[sh] * ip.max_en_change_up;
+    if (xxx[0] > edenmin && xxx[0] < edenmax && fret >= -1) ok = true;
+    double tgo = t_from_urho(*cx.op, ppp[UU], ppp[RHO], cx.gam);
+    double tgn = t_from_urho(*cx.op, pp[UU], pp[RHO], cx.gam);
+    if (ok && (tgn > tgo * ip.max_tgas_change || tgn < tgo / ip.max_tgas_change)) ok = false;
+    double trn = lte_t_from_e(*cx.op, pp[EE]);
+    double tro = lte_t_from_e(*cx.op, ppp[EE]);
+    if (ok && (trn > tro * ip.max_trad_change || trn < tro / ip.max_trad_change)) ok = false;
+    if (ok) break;
+    if (xxx[0] <= 0.0) {
+      xiapp *= ppp[sh] / (ppp[sh] + kzm_fabs(xxx[0]));
+      xiapp *= 1.0e-1;
+    } else {
+      xiapp /= ip.damping_factor;
+    }
+    if (xiapp < ip.max_damping) {
+      failed = true;
+      break;
+    }
+  }
+  if (!failed) {
+    double f3[4];
+    f3[0] = kzm_fabs((pp[sh] - ppp[sh]) / ppp[sh]);
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      double d = pp[i + sh] - ppp[i + sh];
+      f3[i] = kzm_fabs(d / kzm_fmax(ip.eps, kzm_fabs(ppp[i + sh])));
+    }
+    double cr = (we == WE_ENTROPY) ? ip.convrelentr : ip.convrel;
+    double cre = (we == WE_ENTROPY) ? ip.convrelentrerr : ip.convrelerr;
+    bool cc = true;
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib)
+      if (f3[ib] > cr) cc = false;
+    if (cc || errbase <= cre) return iter > ip.maxiter ? -1 : 1;
+  }
+  if (iter > ip.maxiter || failed) return -1;
+  return 0;
+}
+template <int WP>
+__device__ static __forceinline__ bool solve_4d_prim_core(const double *uu00, const LagOpac &lag, const State0 &s0,
+                                                          const double *pp0in, double dt, const ImpCtx &cx, int we,
+                                                          int wf, double *pp_out, int *iters_out) {
+  NewtonState n;
+  newton_start<WP>(n, uu00, pp0in, cx);
+  int r;
+  do {
+    r = newton_step<WP>(n, uu00, lag, s0, dt, cx, we, wf);
+  } while (r == 0);
+  if (r < 0) return false;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) pp_out[i] = n.pp[i];
+  *iters_out = n.iter;
   return true;
 }
 

@@ -1041,6 +1041,12 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, D
 //                  that remain and those freed as waves finish
 //   k_imp_slow     rungs 1..5 (and damped-opacity levels) for the cells the
 //                  first rung did not solve, gathered into a list
+// Batches, whose cells need from one to forty iterations per rung, run the
+// first rung and then each later rung of level 0 as persistent work-stealing
+// launches (k_imp_rung0_persist, k_imp_rung_persist): every lane holds one
+// cell's Newton state and takes the next cell of the queue when its cell is
+// solved or has failed, so a wave's time is its lanes' total iterations, not
+// 64 times its slowest cell's; k_imp_slow then takes the damped levels.
 // The sequence of attempts per cell is the reference ladder's; inputs to
 // every rung are the stored reference-state values, so results equal the
 // single-pass ladder bit for bit.
@@ -1053,6 +1059,12 @@ struct ImpWork {
   int32_t *list;                  // [2][n] cells by starting block (MHD, RAD)
   unsigned long long *nslow;      // length of `slow`
   unsigned long long *ncnt;       // [2] lengths of the two lists
+  // batches (null for grids): the cells the persistent first rung left,
+  // rl = [2][n] by starting block (MHD, RAD) with lengths rc = [2], and the
+  // queue claim positions qpos (first rung RAD, MHD; the later-rung ladder)
+  int32_t *rl;
+  unsigned long long *rc;
+  unsigned long long *qpos;
 };
 // The cells of one run's domain (opImplicit, sim.zig:1736-1776): polar rows
 // are skipped, dt is a step scalar, a solved cell stores u and p and clears
@@ -1131,7 +1143,13 @@ struct BatchCells {
 template <class S>
 __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(S s, ImpWork w, kz_physics ph) {
   long t = TID;
-  if (t == 0) *w.nslow = 0;  // no appends run concurrently with this kernel
+  if (t == 0) {  // no appends run concurrently with this kernel
+    *w.nslow = 0;
+    if (w.qpos) {
+      w.rc[0] = w.rc[1] = 0;
+      for (int k = 0; k < 3; ++k) w.qpos[k] = 0;
+    }
+  }
   const long nd = s.count();
   if (t >= nd) return;
   if (s.skip(ph, t)) return;
@@ -1219,8 +1237,174 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0(S s, ImpW
   if (b < brad) rung0_cell<S, WP_RAD>(s, w, ph, b * 64 + threadIdx.x);
   else rung0_cell<S, WP_MHD>(s, w, ph, (b - brad) * 64 + threadIdx.x);
 }
+// Where a cell of the persistent first-rung launch goes when its Newton
+// ends: solved (recorded with the rung's attempt) or to the later rungs'
+// list of its starting block.
 template <class S>
-__global__ void __launch_bounds__(64) k_imp_slow(S s, ImpWork w, kz_physics ph) {
+struct RungDone {
+  const S *s;
+  const ImpWork *w;
+  const kz_physics *ph;
+  int ir, which, wp, we, wf;
+  __device__ void operator()(long t, const Geo &gm, const NewtonState &n, bool ok) const {
+    if (ok) {
+      s->solved(*ph, t, gm, n.pp, n.iter, ir, wp, we, wf);
+      return;
+    }
+    const long nd = s->count();
+    unsigned long long k = atomicAdd(&w->rc[which], 1ULL);
+    w->rl[which * nd + k] = (int32_t)t;
+  }
+};
+// Work stealing over one queue (cells list[0 .. qlen)) with Newton variant
+// WP at opdamp level 0: each lane holds one cell; the lanes without one claim
+// the next queue entries together (one atomic per wave and refill) until the
+// queue is exhausted; the loop ends when no lane holds a cell.  Every Newton
+// ends within maxiter + 1 steps and the queue is finite, so every wave exits.
+template <class S, int WP>
+__device__ __forceinline__ void drain_queue(const S &s, const ImpWork &w, const kz_physics &ph, const int32_t *list,
+                                            long qlen, unsigned long long *qpos, int we, int wf,
+                                            const RungDone<S> &done) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ULL << lane) - 1ULL;
+  const long nd = s.count();
+  bool has = false, exhausted = qlen <= 0;
+  long t = 0;
+  double dt = 0;
+  NewtonState ns;
+  double uu00[NV];
+  State0 s0;
+  LagOpac lag;
+  Geo gm;
+  for (;;) {
+    const unsigned long long need = __ballot(!has && !exhausted);
+    if (need) {
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      const int cnt = __popcll(need);
+      long base = 0;
+      if (lane == leader) base = (long)atomicAdd(qpos, (unsigned long long)cnt);
+      base = __shfl(base, leader);
+      if (!has && !exhausted) {
+        const long q = base + __popcll(need & below);
+        if (q < qlen && ph.implicit.opdamp_maxlevels < 0) {
+          done(list[q], gm, ns, false);  // the ladder has no level to run (implicit.zig:961)
+        } else if (q < qlen) {
+          t = list[q];
+          double pp0[NV];
+          imp_load(w, nd, t, uu00, pp0, s0, lag);
+          s.geo(t, gm);
+          dt = s.dt(t);
+          const ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+          newton_start<WP>(ns, uu00, pp0, cx);
+          has = true;
+        }
+      }
+      if (base + cnt >= qlen) exhausted = true;  // the same for every lane of the wave
+    }
+    if (!__ballot(has)) break;
+    if (has) {
+      const ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+      const int r = newton_step<WP>(ns, uu00, lag, s0, dt, cx, we, wf);
+      if (r != 0) {
+        done(t, gm, ns, r > 0);
+        has = false;
+      }
+    }
+  }
+}
+// first rung over both starting-block lists, RAD-start queue first
+template <class S>
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_rung0_persist(S s, ImpWork w, kz_physics ph) {
+  const long nd = s.count();
+  const RungDone<S> dr{&s, &w, &ph, 0, 1, WP_RAD, WE_ENERGY, WF_LAB};
+  drain_queue<S, WP_RAD>(s, w, ph, w.list + nd, (long)w.ncnt[1], &w.qpos[0], WE_ENERGY, WF_LAB, dr);
+  const RungDone<S> dm{&s, &w, &ph, 0, 0, WP_MHD, WE_ENERGY, WF_LAB};
+  drain_queue<S, WP_MHD>(s, w, ph, w.list, (long)w.ncnt[0], &w.qpos[1], WE_ENERGY, WF_LAB, dm);
+}
+// Rungs 1..5 of level 0 (implicit.zig:961-968) over the cells the first
+// rung left (both starting-block lists, MHD start then RAD start), work
+// stealing as drain_queue but with each lane walking its own cell up the
+// ladder: when an attempt fails the lane starts the next rung's Newton (whose
+// primitive block may differ from its neighbours', the two variants then run
+// one after the other), so the long ladders of different rungs overlap
+// instead of each rung waiting for its slowest cell.  Cells past rung 5 go
+// to the slow list.
+template <class S>
+__global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_RUNG0) k_imp_ladder_persist(S s, ImpWork w, kz_physics ph) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ULL << lane) - 1ULL;
+  const long nd = s.count();
+  const long n0 = (long)w.rc[0], qlen = n0 + (long)w.rc[1];
+  bool has = false, exhausted = qlen <= 0;
+  long t = 0;
+  int sw = WP_MHD, ir = 1, wp = WP_MHD, we = WE_ENERGY, wf = WF_FF;
+  double dt = 0;
+  NewtonState ns;
+  double uu00[NV], pp0[NV];
+  State0 s0;
+  LagOpac lag;
+  Geo gm;
+  for (;;) {
+    const unsigned long long need = __ballot(!has && !exhausted);
+    if (need) {
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      const int cnt = __popcll(need);
+      long base = 0;
+      if (lane == leader) base = (long)atomicAdd(&w.qpos[2], (unsigned long long)cnt);
+      base = __shfl(base, leader);
+      if (!has && !exhausted) {
+        const long q = base + __popcll(need & below);
+        if (q < qlen) {
+          sw = q < n0 ? WP_MHD : WP_RAD;
+          t = q < n0 ? w.rl[q] : w.rl[nd + (q - n0)];
+          if (ph.implicit.opdamp_maxlevels < 0) {  // no level to run (implicit.zig:961)
+            unsigned long long k = atomicAdd(w.nslow, 1ULL);
+            w.slow[k] = (int32_t)t;
+          } else {
+            imp_load(w, nd, t, uu00, pp0, s0, lag);
+            s.geo(t, gm);
+            dt = s.dt(t);
+            ir = 0;  // the next attempt is rung 1
+            has = true;
+          }
+        }
+      }
+      if (base + cnt >= qlen) exhausted = true;  // the same for every lane of the wave
+    }
+    if (!__ballot(has)) break;
+    if (has) {
+      const ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+      int r = -1;
+      if (ir > 0) {
+        r = (wp == WP_MHD) ? newton_step<WP_MHD>(ns, uu00, lag, s0, dt, cx, we, wf)
+                           : newton_step<WP_RAD>(ns, uu00, lag, s0, dt, cx, we, wf);
+      }
+      if (r > 0) {
+        s.solved(ph, t, gm, ns.pp, ns.iter, ir, wp, we, wf);
+        has = false;
+      } else if (r < 0) {
+        if (ir == 5) {
+          unsigned long long k = atomicAdd(w.nslow, 1ULL);
+          w.slow[k] = (int32_t)t;
+          has = false;
+        } else {  // the next rung's attempt from the stored reference state
+          ir += 1;
+          const int ot = (sw == WP_RAD) ? WP_MHD : WP_RAD;
+          wp = (ir == 1 || ir == 4) ? sw : ot;
+          we = (ir >= 4) ? WE_ENTROPY : WE_ENERGY;
+          wf = (ir == 2) ? WF_LAB : WF_FF;
+          if (wp == WP_MHD) newton_start<WP_MHD>(ns, uu00, pp0, cx);
+          else newton_start<WP_RAD>(ns, uu00, pp0, cx);
+        }
+      }
+    }
+  }
+}
+
+// The rest of the ladder over `slow` from opdamp level `first_level` (0:
+// rungs 1..5 of level 0 first; 1: after the persistent rung launches).
+template <class S>
+__global__ void __launch_bounds__(64) k_imp_slow(S s, ImpWork w, kz_physics ph, int first_level) {
   long i = TID;
   // the first-rung lists have been consumed; clear them for the next pass
   if (i == 0) w.ncnt[0] = w.ncnt[1] = 0;
@@ -1238,7 +1422,7 @@ __global__ void __launch_bounds__(64) k_imp_slow(S s, ImpWork w, kz_physics ph) 
   double dt = s.dt(t);
   const kz_implicit_params &ip = ph.implicit;
   ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ip, 1.0};
-  for (int level = 0; level <= ip.opdamp_maxlevels; ++level) {
+  for (int level = first_level; level <= ip.opdamp_maxlevels; ++level) {
     double od = 1.0;
     for (int k = 0; k < level; ++k) od /= ip.opdamp_factor;
     cx.opdamp = od;
@@ -2264,7 +2448,8 @@ static void launch_rijvisc(kz_sim *s, int dt_slot) {
 }
 
 static ImpWork imp_work(const Dev &d) {
-  return ImpWork{d.imp_uu00, d.imp_pp0, d.imp_aux, d.imp_slow, d.imp_list, d.imp_nslow, d.imp_ncnt};
+  return ImpWork{d.imp_uu00, d.imp_pp0, d.imp_aux, d.imp_slow, d.imp_list, d.imp_nslow, d.imp_ncnt, nullptr, nullptr,
+                 nullptr};
 }
 
 static void launch_op_implicit(kz_sim *s, int dt_slot) {
@@ -2276,7 +2461,7 @@ static void launch_op_implicit(kz_sim *s, int dt_slot) {
     const ImpWork w = imp_work(s->d);
     k_imp_prep<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
     k_imp_rung0<GridCells><<<nb + 1, 64, 0, s->stream>>>(cells, w, s->ph);
-    k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
+    k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 0);
   }
   if (s->ph.do_fixups && s->ph.do_radimp_fixups)
     k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);
@@ -2838,6 +3023,7 @@ struct kz_implicit_batch {
   std::vector<void *> allocs;
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool has_input = false;
+  unsigned persist_blocks = 1024;
 };
 
 static int balloc(kz_implicit_batch *b, void **p, size_t bytes) {
@@ -2887,6 +3073,15 @@ kz_implicit_batch *kz_implicit_batch_create(int64_t n, const kz_physics *ph, int
   e |= balloc(b, (void **)&b->w.list, (size_t)2 * n * sizeof(int32_t));
   e |= balloc(b, (void **)&b->w.nslow, sizeof(unsigned long long));
   e |= balloc(b, (void **)&b->w.ncnt, 2 * sizeof(unsigned long long));
+  e |= balloc(b, (void **)&b->w.rl, (size_t)2 * n * sizeof(int32_t));
+  e |= balloc(b, (void **)&b->w.rc, 2 * sizeof(unsigned long long));
+  e |= balloc(b, (void **)&b->w.qpos, 3 * sizeof(unsigned long long));
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      e |= KZ_E_HIP;
+    b->persist_blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);  // one wave per SIMD
+  }
   for (auto &x : b->ev) e |= hipEventCreate(&x) != hipSuccess;
   if (e) {
     *err = e < 0 ? e : KZ_E_HIP;
@@ -2941,8 +3136,10 @@ int kz_implicit_batch_solve(kz_implicit_batch *b, double *ms) {
   const unsigned nb = nblk(n, 64);
   HIPCHK(hipEventRecord(b->ev[0], b->stream));
   k_imp_prep<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
-  k_imp_rung0<BatchCells><<<nb + 1, 64, 0, b->stream>>>(cells, b->w, b->ph);
-  k_imp_slow<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  const unsigned np = b->persist_blocks < nb + 1 ? b->persist_blocks : nb + 1;
+  k_imp_rung0_persist<BatchCells><<<np, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_ladder_persist<BatchCells><<<np, 64, 0, b->stream>>>(cells, b->w, b->ph);
+  k_imp_slow<BatchCells><<<nb, 64, 0, b->stream>>>(cells, b->w, b->ph, 1);
   HIPCHK(hipEventRecord(b->ev[1], b->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(b->stream));

@@ -119,7 +119,7 @@ def c2_cells(n, seed=0x4B5A4D493335355F):
     T = pk.abi_types
     workloads = __import__("koral_zig_amd.workloads", fromlist=["workloads"])
     ph = T.physics_puffy()
-    pp, geom, dt = workloads.implicit_batch_inputs(n, seed, ph, lambda p, g: kzo.radstate(p, g, ph.gam, ph.opac, 0)[:, 9])
+    pp, geom, dt = workloads.implicit_batch_inputs(n, seed, ph, lambda p, g: kzo.opac_entry(p, g, ph.gam, ph.opac)[:, 3])
     return pp, geom, dt, ph
 
 

@@ -28,7 +28,7 @@ def _count_mod():
 
 def _cells(n, seed=3):
     ph = T.physics_puffy()
-    pp, geom, dt = workloads.implicit_batch_inputs(n, seed, ph, lambda p, g: kzo.radstate(p, g, ph.gam, ph.opac, 0)[:, 9])
+    pp, geom, dt = workloads.implicit_batch_inputs(n, seed, ph, lambda p, g: kzo.opac_entry(p, g, ph.gam, ph.opac)[:, 3])
     return ph, pp, geom, dt
 
 
