@@ -95,12 +95,13 @@ def cpu_baseline(grid, phys, tab, p, u, clock, steps, threads=1):
     sim = kzo.Sim(grid, phys, tab)
     sim.set_state(p, u)
     sim.set_time(clock["t"], clock["tstepdenmax"], clock["tstepdenmin"])
-    t0 = time.perf_counter()
+    per = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         sim.step()
-    dt = time.perf_counter() - t0
+        per.append(time.perf_counter() - t0)
     kzo.set_threads(1)
-    return grid.nx * grid.ny * grid.nz * steps / dt, dt
+    return grid.nx * grid.ny * grid.nz / float(np.median(per)), float(np.sum(per))
 
 
 def bench_implicit_batch(args):
@@ -293,33 +294,36 @@ def cpu_baseline_fresh(grid, phys, mp, seed, rho_u_only, steps, threads=1):
     sim = kzo.Sim(grid, phys, tab)
     host.init_puffy(sim, grid, tab, phys, seed=seed, rho_u_only=rho_u_only)
     sim.step()  # the startup step (tiny dt) is not timed
-    t0 = time.perf_counter()
+    per = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         sim.step()
-    dt = time.perf_counter() - t0
+        per.append(time.perf_counter() - t0)
     kzo.set_threads(1)
-    return grid.nx * grid.ny * grid.nz * steps / dt, dt
+    return grid.nx * grid.ny * grid.nz / float(np.median(per)), float(np.sum(per))
 
 
 def cpu_entry(v1, s1, steps1, vt, st, stepst, nt, what):
     """The cpu_baseline object: the all-cores figure as `value`, the
-    single-thread one beside it."""
+    single-thread one beside it; both from the median step time."""
     log(f"cpu baseline: 1 thread {v1:.4e} zone-cycles/s ({s1:.1f}s), {nt} threads {vt:.4e} ({st:.1f}s)")
     return {"value": vt, "unit": "zone-cycles/s", "cores": nt, "kind": "port",
-            "sample": f"{stepst} CFL steps of {what} on {nt} threads ({st:.1f}s); the same oracle, bitwise the "
-                      f"serial result, at 1 thread over {steps1} steps ({s1:.1f}s): {v1:.4e} zone-cycles/s",
+            "sample": f"median of {stepst} CFL steps of {what} on {nt} threads ({st:.1f}s); the same oracle, "
+                      f"bitwise the serial result, on 1 thread, median of {steps1} steps ({s1:.1f}s): "
+                      f"{v1:.4e} zone-cycles/s",
             "value_1thread": v1}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=384)
     ap.add_argument("--ny", type=int, default=360)
     ap.add_argument("--seed", type=int, default=SEED)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="oracle steps in the single-thread CPU baseline sample (all cores: at least 20; 0: skip)")
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--workload", choices=["puffy", "wedge3d", "implicit-batch", "run-server"], default="puffy",
                     help="puffy: the 384x360 step (default); wedge3d: the 3D wedge; implicit-batch: the "
@@ -461,7 +465,7 @@ def run_grid(args, grid, phys, mp, metric, label, rho_u_only, cpu_grid):
 
     cpu = None
     nt = host_threads()
-    kt = max(args.cpu_steps, 4)
+    kt = max(args.cpu_steps, 20)
     if args.cpu_steps > 0 and cpu_grid is None:
         p, u, _ = sim.get_state()
         c = sim.counters()
