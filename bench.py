@@ -409,11 +409,13 @@ def run_grid(args, grid, phys, mp, metric, label, rho_u_only, cpu_grid):
     # per-class breakdown over one step each (outside the timed region)
     classes = ["face_flux", "implicit", "u2p", "update", "wavespeeds", "ct", "bc"]
     per_step = {}
+    per_launch = {}
     for k in classes:
         sim.time_kernel(k)
         sim.step_many(1)
         ms, n = sim.kernel_time()
         per_step[k] = ms
+        per_launch[k] = ms / max(n, 1)
     sim.time_kernel(None)
     dom = max(per_step, key=per_step.get)
     log("per-step kernel ms: " + ", ".join(f"{k} {v:.3f}" for k, v in per_step.items()) + f" -> dominant {dom}")
@@ -453,6 +455,17 @@ def run_grid(args, grid, phys, mp, metric, label, rho_u_only, cpu_grid):
     roof["kernel"] = dom
     roof["avg_launch_ms"] = avg_ms
     roof["launches"] = n_dom
+    # the HBM-bound classes against their algorithmic bytes per launch, from
+    # the per-class timings of one step before the timed region
+    comps = {}
+    for k, units in (("face_flux", face_units(grid)), ("update", zones), ("u2p", zones)):
+        ms_l = per_launch[k]
+        if ms_l > 0:
+            gbs = BYTES_PER_UNIT[k] * units / (ms_l * 1e-3) / 1e9
+            comps[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "avg_launch_ms": ms_l, "bytes_per_unit": BYTES_PER_UNIT[k],
+                        "units_per_launch": units}
+    roof["components"] = comps
 
     # step latency: individually synchronised steps
     lat = []
