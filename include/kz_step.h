@@ -100,6 +100,13 @@ int kz_sim_sync(kz_sim *s);
  * are host-driven. */
 int kz_sim_capture_steps(kz_sim *s, int nsteps);
 int kz_sim_replay(kz_sim *s);
+/* Launch shape of the implicit pass (no counterpart in the reference; the
+ * results are identical either way): 0 chooses by size (default), 1 one
+ * first-rung launch over both starting-block lists plus one launch for the
+ * later rungs, 2 persistent work-stealing launches (each lane takes the next
+ * cell when its Newton ends), which pay when the cells span many rounds of
+ * waves. */
+int kz_sim_set_implicit_launch(kz_sim *s, int mode);
 /* 0 when everything enqueued on the sim's stream has finished, 1 while
  * work is pending (a non-blocking poll; kz_sim_sync reports errors). */
 int kz_sim_query(kz_sim *s);

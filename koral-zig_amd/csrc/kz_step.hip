@@ -67,6 +67,9 @@ struct Dev {
   unsigned long long *imp_nslow;
   int32_t *imp_list;             // [2][nd] cells by starting block (MHD, RAD)
   unsigned long long *imp_ncnt;  // [2] lengths of the two lists
+  int32_t *imp_rl;                // [2][nd] cells the persistent first rung left
+  unsigned long long *imp_rc;     // [2] their lengths
+  unsigned long long *imp_qpos;   // [3] queue claim positions
   // the u2p fixup list
   unsigned *sync;     // [SY_NFIX] u2p fixup list length
   int32_t *fix_list;  // domain cells flagged for a u2p fixup
@@ -2074,6 +2077,8 @@ struct kz_sim {
   size_t dev_bytes = 0;
   double *scal_buf = nullptr;  // per-cell terms of the scalar diagnostics (allocated on first use)
   double wedge = 0;            // maxz - minz of the caller's grid
+  unsigned persist_blocks = 1024;  // one wave per SIMD for the persistent implicit launches
+  int imp_launch = 0;              // 0 by size, 1 merged first rung, 2 persistent
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_steps = 0;
@@ -2178,6 +2183,11 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   kz_sim *s = new kz_sim();
   s->ph = *ph;
   s->wedge = gr->maxz - gr->minz;
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      s->persist_blocks = (unsigned)(4 * ncu);
+  }
   GridD &G = s->g;
   G.nx = gr->nx; G.ny = gr->ny; G.nz = gr->nz; G.ng = gr->ng;
   G.ngx = gr->nx > 1 ? gr->ng : 0;
@@ -2233,6 +2243,9 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   e |= dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long));
   e |= dalloc(s, (void **)&D.imp_list, (size_t)2 * nd * sizeof(int32_t));
   e |= dalloc(s, (void **)&D.imp_ncnt, 2 * sizeof(unsigned long long));
+  e |= dalloc(s, (void **)&D.imp_rl, (size_t)2 * nd * sizeof(int32_t));
+  e |= dalloc(s, (void **)&D.imp_rc, 2 * sizeof(unsigned long long));
+  e |= dalloc(s, (void **)&D.imp_qpos, 3 * sizeof(unsigned long long));
   e |= dalloc(s, (void **)&D.sync, SY_N * sizeof(unsigned));
   e |= dalloc(s, (void **)&D.fix_list, (size_t)nd * sizeof(int32_t));
   if (ph->radviscosity || ph->dynamo) e |= dalloc(s, (void **)&D.bl, (size_t)G.ncell * BL_NC * sizeof(double));
@@ -2447,7 +2460,10 @@ static void launch_rijvisc(kz_sim *s, int dt_slot) {
   k_rijvisc<<<nblk(nw, 64), 64, 0, s->stream>>>(g, s->d, s->ph, dt_slot, s->rhor);
 }
 
-static ImpWork imp_work(const Dev &d) {
+static ImpWork imp_work(const Dev &d, bool persist) {
+  if (persist)
+    return ImpWork{d.imp_uu00, d.imp_pp0, d.imp_aux, d.imp_slow, d.imp_list, d.imp_nslow, d.imp_ncnt, d.imp_rl,
+                   d.imp_rc, d.imp_qpos};
   return ImpWork{d.imp_uu00, d.imp_pp0, d.imp_aux, d.imp_slow, d.imp_list, d.imp_nslow, d.imp_ncnt, nullptr, nullptr,
                  nullptr};
 }
@@ -2458,10 +2474,21 @@ static void launch_op_implicit(kz_sim *s, int dt_slot) {
     TimedLaunch tl(s, KZ_K_IMPLICIT);
     unsigned nb = nblk(ndom(s), 64);
     const GridCells cells{s->g, s->d, dt_slot};
-    const ImpWork w = imp_work(s->d);
+    // persistent launches pay where the cells span many rounds of waves
+    // (192x192x64: 11.4 -> 10.6 ms of implicit work per step); at 384x360
+    // (two rounds) the merged launch is faster (0.85 against 0.91 ms)
+    const bool persist = s->imp_launch == 2 || (s->imp_launch == 0 && ndom(s) >= 8L * 64 * (long)s->persist_blocks);
+    const ImpWork w = imp_work(s->d, persist);
     k_imp_prep<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
-    k_imp_rung0<GridCells><<<nb + 1, 64, 0, s->stream>>>(cells, w, s->ph);
-    k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 0);
+    if (persist) {
+      const unsigned np = s->persist_blocks < nb + 1 ? s->persist_blocks : nb + 1;
+      k_imp_rung0_persist<GridCells><<<np, 64, 0, s->stream>>>(cells, w, s->ph);
+      k_imp_ladder_persist<GridCells><<<np, 64, 0, s->stream>>>(cells, w, s->ph);
+      k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 1);
+    } else {
+      k_imp_rung0<GridCells><<<nb + 1, 64, 0, s->stream>>>(cells, w, s->ph);
+      k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 0);
+    }
   }
   if (s->ph.do_fixups && s->ph.do_radimp_fixups)
     k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);
@@ -2622,6 +2649,12 @@ int kz_sim_replay(kz_sim *s) {
   if (!s->graph_exec) return KZ_E_INVALID_CONFIG;
   HIPCHK(hipGraphLaunch(s->graph_exec, s->stream));
   s->nstep += (uint64_t)s->graph_steps;
+  return 0;
+}
+
+int kz_sim_set_implicit_launch(kz_sim *s, int mode) {
+  if (mode < 0 || mode > 2) return KZ_E_INVALID_CONFIG;
+  s->imp_launch = mode;
   return 0;
 }
 

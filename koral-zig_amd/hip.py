@@ -43,6 +43,7 @@ _SIGS = {
     "kz_sim_capture_steps": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_replay": (C.c_int, [C.c_void_p]),
     "kz_sim_query": (C.c_int, [C.c_void_p]),
+    "kz_sim_set_implicit_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_sim_scalars": (C.c_int, [C.c_void_p, C.c_double, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_void_p,
                                  C.POINTER(T.kz_scalar_row)]),
@@ -243,6 +244,10 @@ class Sim:
     def replay(self):
         """Enqueue the captured steps without waiting (sync() waits)."""
         _chk(lib().kz_sim_replay(self.h), "kz_sim_replay")
+
+    def set_implicit_launch(self, mode):
+        """0 by size, 1 merged first-rung launch, 2 persistent work stealing."""
+        _chk(lib().kz_sim_set_implicit_launch(self.h, int(mode)), "kz_sim_set_implicit_launch")
 
     def busy(self):
         """True while enqueued work is still running (non-blocking)."""

@@ -157,3 +157,27 @@ def test_set_time_resume():
     gpu.close()
     gpu2.close()
 
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_implicit_launch_shapes(mode):
+    """Both launch shapes of the implicit pass (one merged first-rung launch;
+    persistent work stealing, which large grids use) give the oracle's steps
+    bit for bit, with radiative viscosity and the dynamo on."""
+    grid = T.grid_puffy(40, 36)
+    phys = T.physics_puffy(radviscosity=True, dynamo=True)
+    tab = host.metric_tables(grid)
+    bl = host.bl_tables(grid)
+    cpu = kzo.Sim(grid, phys, tab)
+    gpu = hip.Sim(grid, phys, tab)
+    gpu.set_implicit_launch(mode)
+    for s in (cpu, gpu):
+        s.set_bl_tables(bl)
+        host.init_puffy(s, grid, tab, phys, seed=23)
+    for _ in range(3):
+        cpu.step()
+    gpu.step_many(3)
+    gpu.sync()
+    assert_same(cpu, gpu, f"launch shape {mode}")
+    assert cpu.counters()["n_radimp_solves"] > 0
+    gpu.close()
