@@ -1,0 +1,159 @@
+"""Occupancy autotuner for the step's heavy kernels.
+
+Each heavy kernel carries an `amdgpu_waves_per_eu` bound (KZ_WPE_* in
+csrc/kz_step.hip): a larger bound caps the registers per lane so that more
+waves share a SIMD and hide each other's latency, at the price of spills.
+The best bound depends on the kernel, the grid and the compiler, so it is
+measured: this module builds one library per candidate bound (only the
+bound of one kernel class differs from the defaults in each), times every
+kernel class of the PUFFY step on the GPU with HIP events in a subprocess
+per library, requires every candidate to leave the state bit for bit where
+the default build leaves it, and reports the fastest bound per class (the
+default unless another is at least 1 % faster).
+
+    python koral-zig_amd/tune.py [--jobs N] [--nx 384 --ny 360] [--out DIR]
+
+from the repository root, on a machine with the GPU.  The defaults in
+kz_step.hip are the bounds this tuner selected.
+"""
+import argparse
+import concurrent.futures as cf
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+
+# kernel class (hip.KERNELS name) -> the bound macro that governs it, with the
+# default and the candidates tried
+KNOBS = {
+    "face_flux": ("KZ_WPE_FACE", 1, (1, 2)),
+    "update": ("KZ_WPE_UPDATE", 2, (1, 2, 3)),
+    "u2p": ("KZ_WPE_U2P", 2, (1, 2, 3)),
+    "wavespeeds": ("KZ_WPE_WS", 1, (1, 2, 4)),
+    "implicit": ("KZ_WPE_PREP", 2, (1, 2, 3)),
+    "implicit_rung0": ("KZ_WPE_RUNG0", 1, (1, 2)),
+}
+CLASSES = ("face_flux", "implicit", "u2p", "update", "wavespeeds", "ct", "bc")
+
+
+def variants():
+    """(name, {macro: value}) for the default build and each single change."""
+    out = [("default", {})]
+    for cls, (macro, default, cands) in KNOBS.items():
+        for v in cands:
+            if v != default:
+                out.append((f"{macro}={v}", {macro: v}))
+    return out
+
+
+def build_variant(name, defs, outdir):
+    sys.path.insert(0, str(HERE))
+    import build as B
+    Path(outdir).mkdir(parents=True, exist_ok=True)
+    lib = Path(outdir) / ("libkz_hip_" + name.replace("=", "_") + ".so")
+    cmd = [B._hipcc(), *B.HIPCC_FLAGS, *[f"-D{k}={v}" for k, v in defs.items()], "-o", str(lib),
+           *map(str, B.HIP_SOURCES)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build of {name} failed:\n{r.stderr[-2000:]}")
+    return lib
+
+
+def measure(lib, nx, ny, steps):
+    """Per-class kernel ms per step, ms per step and a hash of the state
+    after a fixed number of steps, with `lib` loaded in place of the built
+    library."""
+    import importlib.util
+    name = "koral_zig_amd"
+    spec = importlib.util.spec_from_file_location(name, HERE / "__init__.py", submodule_search_locations=[str(HERE)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    from koral_zig_amd import abi_types as T
+    from koral_zig_amd import hip, host
+    hip.LIB_PATH = Path(lib)
+    grid = T.grid_puffy(nx, ny)
+    phys = T.physics_puffy()
+    tab = host.metric_tables(grid)
+    sim = hip.Sim(grid, phys, tab)
+    host.init_puffy(sim, grid, tab, phys, seed=0x4B5A4D493335355F)
+    sim.step_many(3)
+    sim.sync()
+    per = {}
+    for k in CLASSES:
+        sim.time_kernel(k)
+        sim.step_many(2)
+        ms, _ = sim.kernel_time()
+        per[k] = ms / 2
+    sim.time_kernel(None)
+    t0 = time.perf_counter()
+    sim.step_many(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    p, u, f = sim.get_state()
+    h = hashlib.sha1(p.tobytes() + u.tobytes() + f.tobytes()).hexdigest()[:16]
+    sim.close()
+    return {"per": per, "ms_per_step": wall / steps * 1e3, "hash": h}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--nx", type=int, default=384)
+    ap.add_argument("--ny", type=int, default=360)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--out", default=str(HERE / "lib" / "tune"))
+    ap.add_argument("--measure", help=argparse.SUPPRESS)
+    a = ap.parse_args()
+    if a.measure:
+        print(json.dumps(measure(a.measure, a.nx, a.ny, a.steps)), flush=True)
+        return
+    Path(a.out).mkdir(parents=True, exist_ok=True)
+    vs = variants()
+    t0 = time.perf_counter()
+    libs = {}
+    with cf.ThreadPoolExecutor(a.jobs) as ex:
+        futs = {ex.submit(build_variant, n, d, a.out): n for n, d in vs}
+        for fu in cf.as_completed(futs):
+            libs[futs[fu]] = fu.result()
+            print(f"  built {futs[fu]} ({time.perf_counter() - t0:.0f}s)", flush=True)
+    libs = {n: libs[n] for n, _ in vs}
+    print(f"built {len(libs)} variants in {time.perf_counter() - t0:.0f}s", flush=True)
+    res = {}
+    for name, lib in libs.items():
+        r = subprocess.run([sys.executable, str(Path(__file__)), "--measure", str(lib), "--nx", str(a.nx), "--ny",
+                            str(a.ny), "--steps", str(a.steps)], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"measuring {name} failed:\n{r.stderr[-2000:]}")
+        res[name] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"{name:18s} {res[name]['ms_per_step']:.4f} ms/step " +
+              " ".join(f"{k} {v:.4f}" for k, v in res[name]["per"].items()) + f" {res[name]['hash']}", flush=True)
+    ref = res["default"]["hash"]
+    bad = [n for n, r in res.items() if r["hash"] != ref]
+    if bad:
+        raise RuntimeError(f"variants changed the state bits: {bad}")
+    print("every variant leaves the state bit for bit as the default build")
+    pick = {}
+    for cls, (macro, default, cands) in KNOBS.items():
+        col = "implicit" if cls.startswith("implicit") else cls
+        times = {default: res["default"]["per"][col]}
+        for v in cands:
+            if v != default:
+                times[v] = res[f"{macro}={v}"]["per"][col]
+        best = min(times, key=times.get)
+        if times[best] > 0.99 * times[default]:  # keep the default unless 1 % faster (timing noise)
+            best = default
+        pick[macro] = best
+        print(f"{macro:14s} default {default}: " + ", ".join(f"{v} -> {t:.4f} ms" for v, t in sorted(times.items())) +
+              f"  best {best}")
+    print(json.dumps({"selected": pick}))
+
+
+if __name__ == "__main__":
+    main()
