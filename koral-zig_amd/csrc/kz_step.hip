@@ -2124,7 +2124,7 @@ static int dalloc(kz_sim *s, void **p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) return KZ_E_OOM;
   s->allocs.push_back(*p);
   s->dev_bytes += bytes;
-  return hipMemset(*p, 0, bytes) == hipSuccess ? 0 : KZ_E_HIP;
+  return hipMemsetAsync(*p, 0, bytes, s->stream) == hipSuccess ? 0 : KZ_E_HIP;  // ordered before any use
 }
 
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -2256,11 +2256,14 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   }
   if (e) { *err = e < 0 ? e : KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // metric tables
-  e |= hipMemcpy(D.gc, pk_gc.data(), pk_gc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+  e |= hipMemcpyAsync(D.gc, pk_gc.data(), pk_gc.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
   for (int d = 0; d < 3; ++d)
-    e |= hipMemcpy(D.gf[d], pk_gf[d].data(), pk_gf[d].size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
-  e |= hipMemcpy(D.kris, pk_kr.data(), pk_kr.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
-  e |= hipMemcpy(D.bcx, bcx, nbcx * sizeof(double), hipMemcpyHostToDevice) != hipSuccess;
+    e |= hipMemcpyAsync(D.gf[d], pk_gf[d].data(), pk_gf[d].size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+         hipSuccess;
+  e |= hipMemcpyAsync(D.kris, pk_kr.data(), pk_kr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
+  e |= hipMemcpyAsync(D.bcx, bcx, nbcx * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
+  // the host staging vectors must outlive the copies
+  e |= hipStreamSynchronize(s->stream) != hipSuccess;
   if (e) { *err = KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // set_grid minimum cell sizes incl. the mdx quirk (sim.zig:447-466)
   double mdx = -1, mdy = -1, mdz = -1;
@@ -2286,7 +2289,8 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   s->nstep = 0;
   unsigned long long tsd0[2] = {(unsigned long long)__builtin_bit_cast(long long, 0.0),
                                 (unsigned long long)__builtin_bit_cast(long long, 1.0e50)};
-  if (hipMemcpy(D.tsd, tsd0, sizeof tsd0, hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpyAsync(D.tsd, tsd0, sizeof tsd0, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
     *err = KZ_E_HIP;
     kz_sim_destroy(s);
     return nullptr;
@@ -2311,10 +2315,14 @@ static int check_err(kz_sim *s) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   int e = 0;
-  HIPCHK(hipMemcpy(&e, s->d.err, sizeof(int), hipMemcpyDeviceToHost));
+  // stream-ordered copies: no legacy-stream work, which HIP refuses while
+  // another thread captures a graph
+  HIPCHK(hipMemcpyAsync(&e, s->d.err, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   if (e) {
     int z = 0;
-    HIPCHK(hipMemcpy(s->d.err, &z, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(s->d.err, &z, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
   }
   return e;
 }
@@ -2700,10 +2708,11 @@ int kz_sim_init_timestep_guess(kz_sim *s) {
 int kz_sim_set_time(kz_sim *s, double t, double tstepdenmax, double tstepdenmin) {
   HIPCHK(hipStreamSynchronize(s->stream));
   double tt = t;
-  HIPCHK(hipMemcpy(s->d.ss + SS_T, &tt, sizeof tt, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(s->d.ss + SS_T, &tt, sizeof tt, hipMemcpyHostToDevice, s->stream));
   unsigned long long b[2] = {(unsigned long long)__builtin_bit_cast(long long, tstepdenmax),
                              (unsigned long long)__builtin_bit_cast(long long, tstepdenmin)};
-  HIPCHK(hipMemcpy(s->d.tsd, b, sizeof b, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(s->d.tsd, b, sizeof b, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
@@ -2711,9 +2720,10 @@ int kz_sim_counters(kz_sim *s, kz_counters *c) {
   HIPCHK(hipStreamSynchronize(s->stream));
   unsigned long long cnt[8], tsd[2];
   double ss[SS_N];
-  HIPCHK(hipMemcpy(cnt, s->d.cnt, sizeof cnt, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(tsd, s->d.tsd, sizeof tsd, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(ss, s->d.ss, sizeof ss, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(cnt, s->d.cnt, sizeof cnt, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(tsd, s->d.tsd, sizeof tsd, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(ss, s->d.ss, sizeof ss, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   c->t = ss[SS_T];
   c->dt = ss[SS_DT];
   c->own_dt = ss[SS_OWN_DT];

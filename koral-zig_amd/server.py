@@ -85,17 +85,22 @@ def prepare(specs, workers, log=None):
     return out
 
 
-def start_run(spec, prepared, capture_steps=0):
+def start_run(spec, prepared, capture_steps=0, capture_lock=None):
     """A resident run in its initial state: the torus init sequence on the
     GPU from the prepared tables and prepinit, the run's seeded perturbation,
-    and (capture_steps > 0) its step graph."""
+    and (capture_steps > 0) its step graph, captured while holding
+    `capture_lock` when one is given."""
     tab, p0 = prepared
     grid, phys = spec.grid(), spec.phys()
     sim = hip.Sim(grid, phys, tab)
     try:
         host.init_puffy(sim, grid, tab, phys, seed=spec.seed, p0=p0)
         if capture_steps:
-            sim.capture_steps(capture_steps)
+            if capture_lock is None:
+                sim.capture_steps(capture_steps)
+            else:
+                with capture_lock:
+                    sim.capture_steps(capture_steps)
     except Exception:
         sim.close()
         raise
@@ -145,6 +150,9 @@ class RunServer:
         need = footprints(specs, prepared)
         arrivals = collections.deque(sorted(specs, key=lambda s: s.arrival))
         lock = threading.Lock()
+        # a graph capture on one thread and a device-wide operation on the
+        # other (freeing a finished run's memory) must not overlap
+        hip_lock = threading.Lock()
         ready = collections.deque()  # admitted, initialised runs for the stepping loop
         state = {"used": 0, "live": 0, "error": None, "stop": False}
         freed = threading.Condition(lock)
@@ -169,7 +177,7 @@ class RunServer:
                         state["used"] += b
                         state["live"] += 1
                     arrivals.popleft()
-                    sim = start_run(s, prepared.pop(s.rid), self.batch)
+                    sim = start_run(s, prepared.pop(s.rid), self.batch, hip_lock)
                     r = _Resident(s, sim, b)
                     with lock:
                         ready.append(r)
@@ -212,10 +220,11 @@ class RunServer:
                             finished.append(r)
                 for r in finished:
                     resident.remove(r)
-                    r.sim.sync()  # raises on a device-side error of the run
-                    if on_finish is not None:
-                        on_finish(r.spec, r.sim)
-                    r.sim.close()
+                    with hip_lock:
+                        r.sim.sync()  # raises on a device-side error of the run
+                        if on_finish is not None:
+                            on_finish(r.spec, r.sim)
+                        r.sim.close()
                     n_done += 1
                     with freed:
                         state["used"] -= r.bytes
@@ -236,8 +245,9 @@ class RunServer:
             th.join()
             with lock:
                 resident.extend(ready)
-            for r in resident:
-                r.sim.close()
+            with hip_lock:
+                for r in resident:
+                    r.sim.close()
         if state["error"] is not None:
             raise state["error"]
         return {"wall_s": wall, "zone_cycles": zc, "latency_ms": np.asarray(lat),

@@ -104,3 +104,44 @@ def test_server_matches_sequential():
 
     seq = server.serve_sequential(specs, prep2, on_finish=check)
     assert seq["zone_cycles"] == res["zone_cycles"]
+
+
+def test_capture_while_another_thread_syncs():
+    """One thread captures a run's step graph while another waits on,
+    reads and frees a second run: the synchronous calls are stream-ordered
+    on each run's own stream, so neither disturbs the capture, and the
+    captured run still matches the oracle."""
+    import threading
+    grid, phys, tab, cpu, gpu = make_pair(24, 20, seed=33)
+    other = [hip.Sim(grid, phys, tab) for _ in range(3)]
+    for o in other:
+        host.init_puffy(o, grid, tab, phys, seed=34)
+    err = []
+
+    def capture():
+        try:
+            for _ in range(5):
+                gpu.capture_steps(3)
+        except Exception as e:  # reported below
+            err.append(e)
+
+    th = threading.Thread(target=capture)
+    th.start()
+    k = 0
+    while th.is_alive():
+        o = other[k % len(other)]
+        o.step_many(1)
+        o.sync()
+        o.counters()
+        o.get_state()
+        k += 1
+    th.join()
+    for o in other:
+        o.close()
+    assert not err, err
+    gpu.replay()
+    gpu.sync()
+    for _ in range(3):
+        cpu.step()
+    assert_same(cpu, gpu, "replay after concurrent capture")
+    gpu.close()

@@ -181,3 +181,17 @@ def test_implicit_launch_shapes(mode):
     assert_same(cpu, gpu, f"launch shape {mode}")
     assert cpu.counters()["n_radimp_solves"] > 0
     gpu.close()
+
+
+def test_c0_64x64_cfl_step():
+    """The C0 case: 64x64 PUFFY torus with the default physics (radiative
+    viscosity and dynamo off), initAllWith + initTimestepGuess, the seeded
+    1e-4 perturbation of every non-B primitive with ENTR and u recomputed
+    and the boundaries refilled, then one step(cflDt()): GPU and oracle
+    identical in fields, flags, clock and implicit counters."""
+    grid, phys, tab, cpu, gpu = make_pair(64, 64, seed=0x4B5A4D493335355F)
+    assert_same(cpu, gpu, "C0 init")
+    cpu.step()
+    gpu.step()
+    assert_same(cpu, gpu, "C0 step")
+    gpu.close()
