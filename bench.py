@@ -41,20 +41,20 @@ FP64_PEAK_TFLOPS = 78.6
 #    + 4 flags 16                                               = 504 B
 BYTES_PER_UNIT = {"face_flux": 400.0, "update": 1224.0, "u2p": 504.0}
 # FP64 operations of one implicit cell-solve, counted by the oracle built
-# with a counting double (oracle/flops: +1 per + - * / sqrt, transcendentals
-# counted through their own operations).  `python oracle/flops/count.py c1`
+# with a counting double (oracle/flops: +1 per + - * / sqrt, +2 per fused
+# multiply-add, transcendentals counted through their own operations).  `python oracle/flops/count.py c1`
 # over the 136 704 implicit cells of the 384x360 step after its first step:
-# 31 617.5 per solve on average; least-squares fit over the cells
-# flops = 17 034.1 + 10 609.1 * Newton iterations (rms 2.4e3).  The bench
+# 28 291.3 per solve on average; least-squares fit over the cells
+# flops = 14 565.3 + 9 985.4 * Newton iterations (rms 1.7e3).  The bench
 # multiplies the fit by the solves and iterations the device counts.
-IMPLICIT_FLOPS_PER_SOLVE = 17034.1
-IMPLICIT_FLOPS_PER_ITER = 10609.1
+IMPLICIT_FLOPS_PER_SOLVE = 14565.3
+IMPLICIT_FLOPS_PER_ITER = 9985.4
 # The independent-cell batch spends most of its operations in the later
 # rungs and in failing ladders, which the fit above does not describe:
 # `python oracle/flops/count.py c2` over 200 000 cells of the same generator
-# counts 152 267.7 operations per cell (first-rung solves 63 516.7, later
-# rungs 768 799.1, failed cells 2 970 648.7).
-BATCH_FLOPS_PER_CELL = 152267.7
+# counts 134 630.8 operations per cell (first-rung solves 55 273.7, later
+# rungs 686 520.3, failed cells 2 625 525.3).
+BATCH_FLOPS_PER_CELL = 134630.8
 # HBM bytes per launch of a kernel class at 384x360, from rocprofv3 runs of
 # FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
 # counter reads for a copy kernel of known size in the same run (k_soa2aos:

@@ -55,7 +55,7 @@ def build_host():
     LIB.mkdir(exist_ok=True)
     out = LIB / "libkz_host.so"
     tmp = LIB / "libkz_host.so.tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fopenmp", "-o", str(tmp),
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-mfma", "-fopenmp", "-o", str(tmp),
           *map(str, HOST_SOURCES)])
     os.replace(tmp, out)
     return out

@@ -7,23 +7,25 @@
  * log/exp/pow/cbrt/log1p.  glibc, ROCm's ocml and Zig's std.math all round
  * those functions differently in the last place, which would make the GPU's
  * iteration counts and fixup flags drift from the CPU path's.  Every routine
- * here is built only from IEEE-754 +, -, *, / and sqrt (all correctly rounded
- * on both targets) and from exact bit manipulation, so with
- * -ffp-contract=off the CPU and the GPU produce bit-identical results.
+ * here is built only from IEEE-754 +, -, *, /, sqrt and fused multiply-add
+ * (all correctly rounded on both targets) and from exact bit manipulation,
+ * so with -ffp-contract=off the CPU and the GPU produce bit-identical
+ * results.
  *
  * Methods (all textbook; no vendor library is reproduced):
  *   exp   — Cody & Waite (1980) additive argument reduction x = k ln2 + r with
  *           a split ln2 whose high part has 32 significant bits (k ln2_hi is
  *           exact for |k| < 2^21), the reduction error carried as a correction
- *           term, Taylor series of degree 13 for exp(r) - 1 - r, scaling by an
- *           exponent-field construction of 2^k.
+ *           term, Taylor series of degree 13 for exp(r) - 1 - r (Horner with
+ *           fused multiply-adds), scaling by an exponent-field construction
+ *           of 2^k.
  *   log   — reduction to m in [sqrt(1/2), sqrt(2)), then
  *           log(m) = 2 atanh(s), s = (m-1)/(m+1), with s carried as a
- *           double-double (Dekker 1971 exact product via Veltkamp splitting),
- *           odd series in s up to s^23.  Result available as a double-double
- *           (used by pow).
+ *           double-double (exact product by fused multiply-add), odd series
+ *           in s up to s^23.  Result available as a double-double (used by
+ *           pow).
  *   pow   — exp(y log x) evaluated in double-double: the product y*log(x) is
- *           formed exactly with Dekker's TwoProduct and the low part enters
+ *           formed exactly (TwoProduct by fused multiply-add) and the low part enters
  *           exp as a first-order correction, so the error does not grow with
  *           |y log x|.
  *   cbrt  — exponent/3 bit-level seed, three Halley iterations, final
@@ -59,12 +61,20 @@ KZM_QUAL double kzm_fmax(double a, double b) { return (a > b || b != b) ? a : b;
 #define KZM_INF (__builtin_inf())
 #define KZM_NAN (__builtin_nan(""))
 
+/* Fused multiply-add a*b + c with one rounding: correctly rounded by IEEE
+ * 754 on both targets (v_fma_f64 on gfx950; the FMA3 instruction or libm's
+ * fma on the host), so it keeps the two builds bit-identical.  A build may
+ * define KZM_FMA itself (the flop-counting build of the oracle does). */
+#ifndef KZM_FMA
+#define KZM_FMA(a, b, c) __builtin_fma((a), (b), (c))
+#endif
+
 /* ln 2 = LN2_HI + LN2_LO; LN2_HI has its low 21 mantissa bits clear. */
 #define KZM_LN2_HI 0x1.62e42fee00000p-1
 #define KZM_LN2_LO 0x1.a39ef35793c76p-33
 #define KZM_INV_LN2 0x1.71547652b82fep+0
 
-/* ---- double-double helpers (Knuth TwoSum, Dekker/Veltkamp TwoProduct) --- */
+/* ---- double-double helpers (Knuth TwoSum; TwoProduct by FMA) ------------- */
 KZM_QUAL void kzm_two_sum(double a, double b, double *s, double *e) {
   double ss = a + b;
   double bb = ss - a;
@@ -77,18 +87,11 @@ KZM_QUAL void kzm_fast_two_sum(double a, double b, double *s, double *e) {
   *e = b - (ss - a);
   *s = ss;
 }
-KZM_QUAL void kzm_split(double a, double *hi, double *lo) {
-  double c = 134217729.0 * a; /* 2^27 + 1 */
-  double h = c - (c - a);
-  *hi = h;
-  *lo = a - h;
-}
+/* p + e == a*b exactly (the rounding error of a product is representable
+ * unless it underflows): e = fma(a, b, -p). */
 KZM_QUAL void kzm_two_prod(double a, double b, double *p, double *e) {
-  double ah, al, bh, bl;
   double pp = a * b;
-  kzm_split(a, &ah, &al);
-  kzm_split(b, &bh, &bl);
-  *e = ((ah * bh - pp) + ah * bl + al * bh) + al * bl;
+  *e = KZM_FMA(a, b, -pp);
   *p = pp;
 }
 
@@ -105,22 +108,23 @@ KZM_QUAL double kzm_exp_dd(double x, double xl) {
   int k = (int)(t >= 0.0 ? t + 0.5 : t - 0.5);
   double kd = (double)k;
   double hi = x - kd * KZM_LN2_HI; /* exact: kd*LN2_HI exact, Sterbenz */
-  double lo = kd * KZM_LN2_LO - xl;
+  double lo = KZM_FMA(kd, KZM_LN2_LO, -xl);
   double r = hi - lo;
   double c = (hi - r) - lo; /* r + c == hi - lo to double-double accuracy */
-  /* q = exp(r) - 1 - r = r^2 (1/2! + r/3! + ... + r^11/13!) */
+  /* q = exp(r) - 1 - r = r^2 (1/2! + r/3! + ... + r^11/13!), Horner steps
+   * fused */
   double q = 0x1.6124613a86d09p-33;              /* 1/13! */
-  q = 0x1.1eed8eff8d898p-29 + r * q;             /* 1/12! */
-  q = 0x1.ae64567f544e4p-26 + r * q;             /* 1/11! */
-  q = 0x1.27e4fb7789f5cp-22 + r * q;             /* 1/10! */
-  q = 0x1.71de3a556c734p-19 + r * q;             /* 1/9!  */
-  q = 0x1.a01a01a01a01ap-16 + r * q;             /* 1/8!  */
-  q = 0x1.a01a01a01a01ap-13 + r * q;             /* 1/7!  */
-  q = 0x1.6c16c16c16c17p-10 + r * q;             /* 1/6!  */
-  q = 0x1.1111111111111p-7 + r * q;              /* 1/5!  */
-  q = 0x1.5555555555555p-5 + r * q;              /* 1/4!  */
-  q = 0x1.5555555555555p-3 + r * q;              /* 1/3!  */
-  q = 0.5 + r * q;                               /* 1/2!  */
+  q = KZM_FMA(r, q, 0x1.1eed8eff8d898p-29);             /* 1/12! */
+  q = KZM_FMA(r, q, 0x1.ae64567f544e4p-26);             /* 1/11! */
+  q = KZM_FMA(r, q, 0x1.27e4fb7789f5cp-22);             /* 1/10! */
+  q = KZM_FMA(r, q, 0x1.71de3a556c734p-19);             /* 1/9!  */
+  q = KZM_FMA(r, q, 0x1.a01a01a01a01ap-16);             /* 1/8!  */
+  q = KZM_FMA(r, q, 0x1.a01a01a01a01ap-13);             /* 1/7!  */
+  q = KZM_FMA(r, q, 0x1.6c16c16c16c17p-10);             /* 1/6!  */
+  q = KZM_FMA(r, q, 0x1.1111111111111p-7);              /* 1/5!  */
+  q = KZM_FMA(r, q, 0x1.5555555555555p-5);              /* 1/4!  */
+  q = KZM_FMA(r, q, 0x1.5555555555555p-3);              /* 1/3!  */
+  q = KZM_FMA(r, q, 0.5);                               /* 1/2!  */
   q = (r * r) * q;
   double y = 1.0 + (r + (q + c * (1.0 + r)));
   if (k > 1023) return (y * 2.0) * kzm_pow2i(k - 1);
@@ -157,16 +161,16 @@ KZM_QUAL void kzm_log_dd(double x, double *hi_out, double *lo_out) {
   double s2 = sh * sh;
   /* tail = 2 s^3 (1/3 + s^2/5 + ... + s^20/23) */
   double p = 0x1.642c8590b2164p-4;                /* 2/23 */
-  p = 0x1.8618618618618p-4 + s2 * p;              /* 2/21 */
-  p = 0x1.af286bca1af28p-4 + s2 * p;              /* 2/19 */
-  p = 0x1.e1e1e1e1e1e1ep-4 + s2 * p;              /* 2/17 */
-  p = 0x1.1111111111111p-3 + s2 * p;              /* 2/15 */
-  p = 0x1.3b13b13b13b14p-3 + s2 * p;              /* 2/13 */
-  p = 0x1.745d1745d1746p-3 + s2 * p;              /* 2/11 */
-  p = 0x1.c71c71c71c71cp-3 + s2 * p;              /* 2/9  */
-  p = 0x1.2492492492492p-2 + s2 * p;              /* 2/7  */
-  p = 0x1.999999999999ap-2 + s2 * p;              /* 2/5  */
-  p = 0x1.5555555555555p-1 + s2 * p;              /* 2/3  */
+  p = KZM_FMA(s2, p, 0x1.8618618618618p-4);              /* 2/21 */
+  p = KZM_FMA(s2, p, 0x1.af286bca1af28p-4);              /* 2/19 */
+  p = KZM_FMA(s2, p, 0x1.e1e1e1e1e1e1ep-4);              /* 2/17 */
+  p = KZM_FMA(s2, p, 0x1.1111111111111p-3);              /* 2/15 */
+  p = KZM_FMA(s2, p, 0x1.3b13b13b13b14p-3);              /* 2/13 */
+  p = KZM_FMA(s2, p, 0x1.745d1745d1746p-3);              /* 2/11 */
+  p = KZM_FMA(s2, p, 0x1.c71c71c71c71cp-3);              /* 2/9  */
+  p = KZM_FMA(s2, p, 0x1.2492492492492p-2);              /* 2/7  */
+  p = KZM_FMA(s2, p, 0x1.999999999999ap-2);              /* 2/5  */
+  p = KZM_FMA(s2, p, 0x1.5555555555555p-1);              /* 2/3  */
   double tail = (s2 * sh) * p;
   double ed = (double)e;
   double a = ed * KZM_LN2_HI; /* exact */

@@ -5,7 +5,7 @@
 // type, so that every +, -, *, / and sqrt executed by kzo_solve_implicit_lab
 // (implicit.zig:927-1004 and everything it calls, including the kz_math.h
 // transcendentals, whose internal operations are counted the same way)
-// adds one to a counter.  Comparisons, selects, fabs, negation and bit
+// adds one to a counter, and every fused multiply-add two.  Comparisons, selects, fabs, negation and bit
 // manipulation add nothing.  The count is the algorithmic FP64 work of a
 // cell-solve: the GPU kernels execute the same operation sequence (they are
 // checked bit for bit against this oracle), so the count times the solves
@@ -52,12 +52,14 @@ inline bool operator==(cd a, cd b) { return a.v == b.v; }
 inline bool operator!=(cd a, cd b) { return a.v != b.v; }
 inline cd csqrt(cd a) { ++nflop; ++nsqrt; return cd(std::sqrt(a.v)); }
 inline cd cfabs(cd a) { return cd(std::fabs(a.v)); }
+inline cd cfma(cd a, cd b, cd c) { nflop += 2; return cd(std::fma(a.v, b.v, c.v)); }
 static_assert(sizeof(cd) == 8, "counting double must keep the layout of double");
 }  // namespace kzc
 
 #define double kzc::cd
 #define __builtin_sqrt(x) kzc::csqrt(x)
 #define __builtin_fabs(x) kzc::cfabs(x)
+#define KZM_FMA(a, b, c) kzc::cfma(a, b, c)
 #define __builtin_inf() kzc::cd(HUGE_VAL)
 #define __builtin_nan(s) kzc::cd(NAN)
 
