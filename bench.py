@@ -204,8 +204,9 @@ def bench_implicit_batch(args):
 
 def bench_run_server(args):
     """Many PUFFY runs served concurrently on one GPU: Poisson arrivals,
-    admission by device footprint against an HBM budget, per-run HIP-graph
-    batches of CFL steps on per-run streams.  value = zone-cycles of all runs
+    admission by device footprint against an HBM budget, HIP-graph batches of
+    CFL steps on per-run streams (or, grouped, one graph of all resident runs
+    with their implicit cells in shared launches).  value = zone-cycles of all runs
     over the wall time from the first arrival to the last completion; the
     same runs stepped one after another with the plain eager loop give the
     sequential figure."""
@@ -219,14 +220,28 @@ def bench_run_server(args):
     prep = server.prepare(specs, workers, log=log)
     log(f"host set-up {time.perf_counter() - t0:.1f}s (metric tables + prepinit, before the clock)")
     seq_prep = {k: v for k, v in prep.items()}  # the sequential pass consumes its own copy
+    grp_prep = {k: v for k, v in prep.items()} if args.server_mode == "both" else None
     budget = int(args.hbm_gib * (1 << 30))
-    srv = server.RunServer(budget, batch_steps=args.batch, log=log)
-    res = srv.serve(specs, prep)
-    lat = res["latency_ms"]
-    value = res["zone_cycles"] / res["wall_s"]
-    log(f"served {res['runs']} runs in {res['wall_s']:.2f}s: {value:.4e} zone-cycles/s, peak resident "
-        f"{res['peak_resident']} ({res['peak_bytes'] / 2**30:.2f} GiB); step latency p50 "
-        f"{np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms")
+
+    def serve(grouped, pr):
+        srv = server.RunServer(budget, batch_steps=args.batch, log=log, grouped=grouped)
+        r = srv.serve(specs, pr)
+        v = r["zone_cycles"] / r["wall_s"]
+        la = r["latency_ms"]
+        log(f"served {r['runs']} runs ({'one run group' if grouped else 'per-run streams'}) in {r['wall_s']:.2f}s: "
+            f"{v:.4e} zone-cycles/s, peak resident {r['peak_resident']} ({r['peak_bytes'] / 2**30:.2f} GiB); step "
+            f"latency p50 {np.percentile(la, 50):.3f} ms p99 {np.percentile(la, 99):.3f} ms"
+            + (f"; {r['regroups']} regroups" if grouped else ""))
+        return r, v, la
+
+    res, value, lat = serve(args.server_mode == "grouped", prep)
+    grouped = None
+    if grp_prep is not None:
+        gr, gv, gl = serve(True, grp_prep)
+        grouped = {"value": gv, "wall_s": gr["wall_s"], "vs_per_run_streams": gv / value,
+                   "step_latency_ms": {"p50": float(np.percentile(gl, 50)), "p99": float(np.percentile(gl, 99)),
+                                       "n": int(gl.size)},
+                   "peak_resident": gr["peak_resident"], "regroups": gr["regroups"]}
     seq = server.serve_sequential(specs, seq_prep, log=log)
     seq_v = seq["zone_cycles"] / seq["step_s"]
     log(f"sequential eager: stepping {seq['step_s']:.2f}s ({seq_v:.4e} zone-cycles/s), {seq['wall_s']:.2f}s "
@@ -267,6 +282,8 @@ def bench_run_server(args):
                    "peak_gib": res["peak_bytes"] / 2**30, "last_arrival_s": specs and max(s.arrival for s in specs)},
         "sequential_eager": {"value": seq_v, "step_s": seq["step_s"], "wall_with_init_s": seq["wall_s"],
                              "speedup": value / seq_v},
+        "server_mode": "one run group" if args.server_mode == "grouped" else "per-run streams",
+        "grouped": grouped,
     }
     print(json.dumps(out), flush=True)
 
@@ -333,6 +350,9 @@ def main():
     ap.add_argument("--run-steps", type=int, default=500, help="run-server: CFL steps per run")
     ap.add_argument("--batch", type=int, default=10, help="run-server: steps per captured graph")
     ap.add_argument("--hbm-gib", type=float, default=8.0, help="run-server: device-memory budget of resident runs")
+    ap.add_argument("--server-mode", choices=["streams", "grouped", "both"], default="streams",
+                    help="run-server: per-run streams, one run group with cross-run implicit launches, or both "
+                         "(value from per-run streams, the group's figures beside it)")
     ap.add_argument("--nx3", type=int, default=192)
     ap.add_argument("--ny3", type=int, default=192)
     ap.add_argument("--nz3", type=int, default=64)

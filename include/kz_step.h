@@ -110,6 +110,35 @@ int kz_sim_set_implicit_launch(kz_sim *s, int mode);
 /* 0 when everything enqueued on the sim's stream has finished, 1 while
  * work is pending (a non-blocking poll; kz_sim_sync reports errors). */
 int kz_sim_query(kz_sim *s);
+
+/* Run groups (no counterpart in the reference, which steps one run per
+ * process): resident runs stepped together, the implicit cells of all of
+ * them solved by one set of implicit launches per implicit pass, each run's
+ * other kernels on its own stream.  Every run ends each step bit for bit
+ * where kz_step_many would leave it.  The runs must agree bitwise in the
+ * physics the implicit solve reads (gam, rad, opac, implicit, has_opac and
+ * the polar-axis rows); grids and spins may differ.  A run must leave the
+ * group (kz_group_set_runs without it) before kz_sim_destroy. */
+typedef struct kz_group kz_group;
+int kz_group_create(kz_group **out);
+void kz_group_destroy(kz_group *g);
+/* Membership: waits for the previous members' work, grows the group's
+ * solve scratch when the cells outgrow it, drops a captured graph, and
+ * records each member's device layout (so call it again after
+ * kz_sim_set_bl_tables on a member). */
+int kz_group_set_runs(kz_group *g, kz_sim *const *sims, int nsims);
+/* as kz_sim_set_implicit_launch, for the group's implicit passes */
+int kz_group_set_implicit_launch(kz_group *g, int mode);
+/* enqueue nsteps CFL steps of every member without waiting */
+int kz_group_step_many(kz_group *g, int nsteps);
+/* capture nsteps steps of every member into one HIP graph; replay enqueues
+ * it on the group's stream (event timing must be off on every member) */
+int kz_group_capture_steps(kz_group *g, int nsteps);
+int kz_group_replay(kz_group *g);
+/* 0 when every enqueued step of every member has finished, 1 while pending */
+int kz_group_query(kz_group *g);
+/* wait, then the first device-side error of any member */
+int kz_group_sync(kz_group *g);
 /* One scalars.dat row of the current state (koral/io/dump.zig:383-411
  * scalarRow on one rank, over io/scalars.zig's reductions): mass, mdot
  * through radial shell ix_h (sign-flipped), radiative and total luminosity

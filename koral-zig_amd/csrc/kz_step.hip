@@ -1143,6 +1143,66 @@ struct BatchCells {
     atomicAdd(&cnt[0], 1ULL);
   }
 };
+// The cells of several runs as one sequence (a run group's implicit pass):
+// run k owns t in [off_k, off_k + ndom_k), each with its own grid, fields,
+// metric, dt slot value and counters.
+struct RunCells {
+  GridD g;
+  Dev d;
+  long off;
+};
+struct MultiCells {
+  const RunCells *r;  // [nr], off ascending, r[0].off == 0
+  int nr;
+  long n;
+  int sslot;
+  __device__ int run(long t) const {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (r[mid].off <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  }
+  __device__ GridCells at(long t, long &tl) const {
+    const int k = run(t);
+    tl = t - r[k].off;
+    return GridCells{r[k].g, r[k].d, sslot};
+  }
+  __device__ long count() const { return n; }
+  __device__ bool skip(const kz_physics &ph, long t) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    return c.skip(ph, tl);
+  }
+  __device__ void load_p(long t, double *pp) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    c.load_p(tl, pp);
+  }
+  __device__ void geo(long t, Geo &gm) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    c.geo(tl, gm);
+  }
+  __device__ double dt(long t) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    return c.dt(tl);
+  }
+  __device__ void solved(const kz_physics &ph, long t, const Geo &gm, const double *ppw, int iters, int ir, int wp,
+                         int we, int wf) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    c.solved(ph, tl, gm, ppw, iters, ir, wp, we, wf);
+  }
+  __device__ void failed(long t) const {
+    long tl;
+    const GridCells c = at(t, tl);
+    c.failed(tl);
+  }
+};
 template <class S>
 __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(S s, ImpWork w, kz_physics ph) {
   long t = TID;
@@ -2476,30 +2536,42 @@ static ImpWork imp_work(const Dev &d, bool persist) {
                  nullptr};
 }
 
+// persistent launches pay where the cells span many rounds of waves
+// (192x192x64: 11.4 -> 10.6 ms of implicit work per step); at 384x360
+// (two rounds) the merged launch is faster (0.85 against 0.91 ms)
+static bool imp_persistent(long n, unsigned persist_blocks, int imp_launch) {
+  return imp_launch == 2 || (imp_launch == 0 && n >= 8L * 64 * (long)persist_blocks);
+}
+// The implicit kernels over the n cells of a cell source (a grid, or the
+// runs of a group); w carries the queue arrays when `persist`.
+extern "C++" template <class S>
+static void launch_implicit(const S &cells, const ImpWork &w, const kz_physics &ph, long n, bool persist,
+                            unsigned persist_blocks, hipStream_t st) {
+  const unsigned nb = nblk(n, 64);
+  k_imp_prep<S><<<nb, 64, 0, st>>>(cells, w, ph);
+  if (persist) {
+    const unsigned np = persist_blocks < nb + 1 ? persist_blocks : nb + 1;
+    k_imp_rung0_persist<S><<<np, 64, 0, st>>>(cells, w, ph);
+    k_imp_ladder_persist<S><<<np, 64, 0, st>>>(cells, w, ph);
+    k_imp_slow<S><<<nb, 64, 0, st>>>(cells, w, ph, 1);
+  } else {
+    k_imp_rung0<S><<<nb + 1, 64, 0, st>>>(cells, w, ph);
+    k_imp_slow<S><<<nb, 64, 0, st>>>(cells, w, ph, 0);
+  }
+}
+static void launch_radimp_fixup(kz_sim *s) {
+  if (s->ph.has_opac && s->ph.do_fixups && s->ph.do_radimp_fixups)
+    k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);
+}
 static void launch_op_implicit(kz_sim *s, int dt_slot) {
   if (!s->ph.has_opac) return;
   {
     TimedLaunch tl(s, KZ_K_IMPLICIT);
-    unsigned nb = nblk(ndom(s), 64);
-    const GridCells cells{s->g, s->d, dt_slot};
-    // persistent launches pay where the cells span many rounds of waves
-    // (192x192x64: 11.4 -> 10.6 ms of implicit work per step); at 384x360
-    // (two rounds) the merged launch is faster (0.85 against 0.91 ms)
-    const bool persist = s->imp_launch == 2 || (s->imp_launch == 0 && ndom(s) >= 8L * 64 * (long)s->persist_blocks);
-    const ImpWork w = imp_work(s->d, persist);
-    k_imp_prep<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph);
-    if (persist) {
-      const unsigned np = s->persist_blocks < nb + 1 ? s->persist_blocks : nb + 1;
-      k_imp_rung0_persist<GridCells><<<np, 64, 0, s->stream>>>(cells, w, s->ph);
-      k_imp_ladder_persist<GridCells><<<np, 64, 0, s->stream>>>(cells, w, s->ph);
-      k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 1);
-    } else {
-      k_imp_rung0<GridCells><<<nb + 1, 64, 0, s->stream>>>(cells, w, s->ph);
-      k_imp_slow<GridCells><<<nb, 64, 0, s->stream>>>(cells, w, s->ph, 0);
-    }
+    const bool persist = imp_persistent(ndom(s), s->persist_blocks, s->imp_launch);
+    launch_implicit(GridCells{s->g, s->d, dt_slot}, imp_work(s->d, persist), s->ph, ndom(s), persist,
+                    s->persist_blocks, s->stream);
   }
-  if (s->ph.do_fixups && s->ph.do_radimp_fixups)
-    k_fixup<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph, KZ_FLAG_RADIMP_FIXUP);
+  launch_radimp_fixup(s);
 }
 
 static void launch_do_correct(kz_sim *s) {
@@ -2580,16 +2652,19 @@ int kz_apply_dynamo(kz_sim *s, double dt) {
 }
 
 // Enqueue one RK2IMEX step (sim.zig:1792-1899) on the sim's stream; no host
-// synchronisation inside.  saveTimesteps (cell_dt) is omitted: nothing on
-// this path reads it.
-static void launch_step(kz_sim *s, double forced, int use_forced) {
+// synchronisation inside (launch_step).  saveTimesteps (cell_dt) is
+// omitted: nothing on this path reads it.  The step is written in three parts
+// around its two implicit passes, so that a group of runs can solve the
+// implicit cells of all its runs in one launch between them.
+static void launch_step_head(kz_sim *s, double forced, int use_forced) {
+  k_begin_step<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, forced, use_forced);  // + ut0 = u
+  if (s->ph.radviscosity) launch_rijvisc(s, SS_DT);                                        // sim.zig:1819-1823
+}
+static void launch_step_mid(kz_sim *s) {
   const GridD &g = s->g;
   Dev &d = s->d;
   hipStream_t st = s->stream;
   const unsigned nb = nblk(ndom(s), 256);
-  k_begin_step<<<nb, 256, 0, st>>>(g, d, forced, use_forced);  // + ut0 = u
-  if (s->ph.radviscosity) launch_rijvisc(s, SS_DT);             // sim.zig:1819-1823
-  launch_op_implicit(s, SS_DTG);                                // U(1)
   k_stage_implicit<<<nb, 256, 0, st>>>(g, d, d.drt1, d.ut0, d.ut1);
   launch_calc_u2p(s, false);
   launch_do_correct(s);
@@ -2599,7 +2674,12 @@ static void launch_step(kz_sim *s, double forced, int use_forced) {
   k_stage_explicit1<<<nb, 256, 0, st>>>(g, d);
   launch_calc_u2p(s, false);
   launch_do_correct(s);
-  launch_op_implicit(s, SS_DTG);                                // U(2)
+}
+static void launch_step_tail(kz_sim *s) {
+  const GridD &g = s->g;
+  Dev &d = s->d;
+  hipStream_t st = s->stream;
+  const unsigned nb = nblk(ndom(s), 256);
   k_stage_implicit<<<nb, 256, 0, st>>>(g, d, d.drt2, d.uforget, d.ut2);
   launch_calc_u2p(s, false);
   launch_do_correct(s);
@@ -2611,6 +2691,13 @@ static void launch_step(kz_sim *s, double forced, int use_forced) {
   launch_do_correct(s);
   launch_set_bc(s);
   k_end_step<<<nb, 256, 0, st>>>(g, d, s->ph);                  // t += dt, updateEntropy
+}
+static void launch_step(kz_sim *s, double forced, int use_forced) {
+  launch_step_head(s, forced, use_forced);
+  launch_op_implicit(s, SS_DTG);  // U(1)
+  launch_step_mid(s);
+  launch_op_implicit(s, SS_DTG);  // U(2)
+  launch_step_tail(s);
 }
 
 int kz_step(kz_sim *s, const double *forced_dt) {
@@ -2679,6 +2766,258 @@ int kz_sim_device_bytes(kz_sim *s, uint64_t *bytes) {
 }
 
 void *kz_sim_stream(kz_sim *s) { return (void *)s->stream; }
+
+// ---------------------------------------------------------------------------
+// Run groups: several resident runs stepped together, the implicit cells of
+// all of them solved by one set of implicit launches per pass.  Each run's
+// other kernels stay on its own stream; the group's stream joins the runs'
+// streams before each implicit pass and forks them again after it.
+struct kz_group {
+  hipStream_t stream = nullptr;
+  std::vector<kz_sim *> sims;
+  std::vector<hipEvent_t> ev;  // one per run: its stream reached the join
+  hipEvent_t ev_fork = nullptr;
+  kz_physics ph{};
+  RunCells *runs = nullptr;  // device table
+  int cap_runs = 0;
+  long n = 0, cap_n = 0;
+  ImpWork w{};
+  std::vector<void *> wallocs;
+  unsigned persist_blocks = 1024;
+  int imp_launch = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
+};
+
+// the fields of kz_physics the implicit kernels read must agree across a group
+static bool group_compatible(const kz_physics &a, const kz_physics &b) {
+  return bits_of(a.gam) == bits_of(b.gam) && !memcmp(&a.rad, &b.rad, sizeof a.rad) &&
+         !memcmp(&a.opac, &b.opac, sizeof a.opac) && !memcmp(&a.implicit, &b.implicit, sizeof a.implicit) &&
+         a.has_opac == b.has_opac && a.correct_polaraxis == b.correct_polaraxis &&
+         a.nccorrectpolar == b.nccorrectpolar;
+}
+
+static void group_free_graph(kz_group *G) {
+  if (G->graph_exec) (void)hipGraphExecDestroy(G->graph_exec);
+  if (G->graph) (void)hipGraphDestroy(G->graph);
+  G->graph_exec = nullptr;
+  G->graph = nullptr;
+  G->graph_steps = 0;
+}
+
+static void group_free_work(kz_group *G) {
+  for (void *p : G->wallocs) (void)hipFree(p);
+  G->wallocs.clear();
+  G->w = ImpWork{};
+  G->cap_n = 0;
+}
+
+static int group_alloc(kz_group *G, void **p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) return KZ_E_HIP;
+  G->wallocs.push_back(*p);
+  if (hipMemsetAsync(*p, 0, bytes, G->stream) != hipSuccess) return KZ_E_HIP;
+  return 0;
+}
+
+int kz_group_create(kz_group **out) {
+  *out = nullptr;
+  kz_group *G = new kz_group();
+  int dev = 0, ncu = 0;
+  if (hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&G->ev_fork, hipEventDisableTiming) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    kz_group_destroy(G);
+    return KZ_E_HIP;
+  }
+  G->persist_blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);
+  *out = G;
+  return 0;
+}
+
+void kz_group_destroy(kz_group *G) {
+  if (!G) return;
+  if (G->stream) (void)hipStreamSynchronize(G->stream);
+  group_free_graph(G);
+  group_free_work(G);
+  if (G->runs) (void)hipFree(G->runs);
+  for (hipEvent_t e : G->ev) (void)hipEventDestroy(e);
+  if (G->ev_fork) (void)hipEventDestroy(G->ev_fork);
+  if (G->stream) (void)hipStreamDestroy(G->stream);
+  delete G;
+}
+
+int kz_group_set_runs(kz_group *G, kz_sim *const *sims, int nsims) {
+  if (nsims < 0 || (nsims > 0 && !sims)) return KZ_E_INVALID_CONFIG;
+  for (int i = 0; i < nsims; ++i) {
+    if (!sims[i] || !group_compatible(sims[0]->ph, sims[i]->ph)) return KZ_E_INVALID_CONFIG;
+    for (int j = 0; j < i; ++j)
+      if (sims[j] == sims[i]) return KZ_E_INVALID_CONFIG;
+  }
+  long n = 0;
+  std::vector<RunCells> tab((size_t)nsims);
+  for (int i = 0; i < nsims; ++i) {
+    tab[i] = RunCells{sims[i]->g, sims[i]->d, n};
+    n += ndom(sims[i]);
+  }
+  if (n > INT32_MAX) return KZ_E_INVALID_CONFIG;
+  // the previous membership's work (and any replay of its graph) ends first
+  for (kz_sim *s : G->sims) HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(G->stream));
+  group_free_graph(G);
+  if (n > G->cap_n) {
+    group_free_work(G);
+    const long c = n + n / 4;
+    int e = 0;
+    e |= group_alloc(G, (void **)&G->w.uu00, (size_t)c * NV * sizeof(double));
+    e |= group_alloc(G, (void **)&G->w.pp0, (size_t)c * NV * sizeof(double));
+    e |= group_alloc(G, (void **)&G->w.aux, (size_t)c * IMP_NAUX * sizeof(double));
+    e |= group_alloc(G, (void **)&G->w.slow, (size_t)c * sizeof(int32_t));
+    e |= group_alloc(G, (void **)&G->w.list, (size_t)2 * c * sizeof(int32_t));
+    e |= group_alloc(G, (void **)&G->w.nslow, sizeof(unsigned long long));
+    e |= group_alloc(G, (void **)&G->w.ncnt, 2 * sizeof(unsigned long long));
+    e |= group_alloc(G, (void **)&G->w.rl, (size_t)2 * c * sizeof(int32_t));
+    e |= group_alloc(G, (void **)&G->w.rc, 2 * sizeof(unsigned long long));
+    e |= group_alloc(G, (void **)&G->w.qpos, 3 * sizeof(unsigned long long));
+    if (e) {
+      group_free_work(G);
+      return KZ_E_HIP;
+    }
+    G->cap_n = c;
+  }
+  if (nsims > G->cap_runs) {
+    if (G->runs) HIPCHK(hipFree(G->runs));
+    G->runs = nullptr;
+    G->cap_runs = 0;
+    HIPCHK(hipMalloc((void **)&G->runs, (size_t)nsims * sizeof(RunCells)));
+    G->cap_runs = nsims;
+  }
+  while ((int)G->ev.size() < nsims) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    G->ev.push_back(e);
+  }
+  if (nsims) {
+    HIPCHK(hipMemcpyAsync(G->runs, tab.data(), (size_t)nsims * sizeof(RunCells), hipMemcpyHostToDevice, G->stream));
+    G->ph = sims[0]->ph;
+  }
+  HIPCHK(hipStreamSynchronize(G->stream));
+  G->sims.assign(sims, sims + nsims);
+  G->n = n;
+  return 0;
+}
+
+int kz_group_set_implicit_launch(kz_group *G, int mode) {
+  if (mode < 0 || mode > 2) return KZ_E_INVALID_CONFIG;
+  G->imp_launch = mode;
+  group_free_graph(G);
+  return 0;
+}
+
+// the group stream waits for every run's stream
+static int group_join(kz_group *G) {
+  for (size_t i = 0; i < G->sims.size(); ++i) {
+    HIPCHK(hipEventRecord(G->ev[i], G->sims[i]->stream));
+    HIPCHK(hipStreamWaitEvent(G->stream, G->ev[i], 0));
+  }
+  return 0;
+}
+// every run's stream waits for the group stream
+static int group_fork(kz_group *G) {
+  HIPCHK(hipEventRecord(G->ev_fork, G->stream));
+  for (kz_sim *s : G->sims) HIPCHK(hipStreamWaitEvent(s->stream, G->ev_fork, 0));
+  return 0;
+}
+static void group_implicit(kz_group *G, int dt_slot) {
+  if (!G->ph.has_opac || G->n == 0) return;
+  const bool persist = imp_persistent(G->n, G->persist_blocks, G->imp_launch);
+  launch_implicit(MultiCells{G->runs, (int)G->sims.size(), G->n, dt_slot}, G->w, G->ph, G->n, persist,
+                  G->persist_blocks, G->stream);
+}
+// nsteps CFL steps of every run: the kernel sequence of launch_step per run,
+// with both implicit passes of each step taken by the group's launches
+static int group_enqueue(kz_group *G, int nsteps) {
+  if (int e = group_fork(G)) return e;
+  for (int k = 0; k < nsteps; ++k) {
+    for (kz_sim *s : G->sims) launch_step_head(s, 0.0, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      if (int e = group_join(G)) return e;
+      group_implicit(G, SS_DTG);  // U(1), U(2)
+      if (int e = group_fork(G)) return e;
+      for (kz_sim *s : G->sims) {
+        launch_radimp_fixup(s);
+        if (pass == 0) launch_step_mid(s);
+        else launch_step_tail(s);
+      }
+    }
+  }
+  if (int e = group_join(G)) return e;
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+static int group_check(kz_group *G) {
+  for (kz_sim *s : G->sims) {
+    if ((s->ph.radviscosity || s->ph.dynamo) && !s->has_bl) return KZ_E_INVALID_CONFIG;
+    if (s->timed != -1) return KZ_E_INVALID_CONFIG;  // event timing is per run, not per group
+  }
+  return 0;
+}
+
+int kz_group_step_many(kz_group *G, int nsteps) {
+  if (nsteps < 0) return KZ_E_INVALID_CONFIG;
+  if (int e = group_check(G)) return e;
+  if (int e = group_enqueue(G, nsteps)) return e;
+  for (kz_sim *s : G->sims) s->nstep += (uint64_t)nsteps;
+  return 0;
+}
+
+int kz_group_capture_steps(kz_group *G, int nsteps) {
+  if (nsteps <= 0) return KZ_E_INVALID_CONFIG;
+  if (int e = group_check(G)) return e;
+  for (kz_sim *s : G->sims) HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(G->stream));
+  group_free_graph(G);
+  HIPCHK(hipStreamBeginCapture(G->stream, hipStreamCaptureModeThreadLocal));
+  const int e = group_enqueue(G, nsteps);
+  hipGraph_t gr = nullptr;
+  const hipError_t ee = hipStreamEndCapture(G->stream, &gr);
+  if (e || ee != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return e ? e : KZ_E_HIP;
+  }
+  G->graph = gr;
+  if (hipGraphInstantiate(&G->graph_exec, G->graph, nullptr, nullptr, 0) != hipSuccess) {
+    group_free_graph(G);
+    return KZ_E_HIP;
+  }
+  G->graph_steps = nsteps;
+  return 0;
+}
+
+// the replay runs after the work already enqueued on the members' streams,
+// and work enqueued on a member's stream afterwards runs after the replay
+int kz_group_replay(kz_group *G) {
+  if (!G->graph_exec) return KZ_E_INVALID_CONFIG;
+  if (int e = group_join(G)) return e;
+  HIPCHK(hipGraphLaunch(G->graph_exec, G->stream));
+  if (int e = group_fork(G)) return e;
+  for (kz_sim *s : G->sims) s->nstep += (uint64_t)G->graph_steps;
+  return 0;
+}
+
+int kz_group_query(kz_group *G) {
+  const hipError_t e = hipStreamQuery(G->stream);
+  if (e == hipSuccess) return 0;
+  if (e == hipErrorNotReady) return 1;
+  return KZ_E_HIP;
+}
+
+int kz_group_sync(kz_group *G) {
+  HIPCHK(hipStreamSynchronize(G->stream));
+  for (kz_sim *s : G->sims)
+    if (int e = check_err(s)) return e;
+  return 0;
+}
 
 int kz_sim_postinit(kz_sim *s, double maxbeta, double *fac_out) {
   unsigned long long *dmax = s->d.cnt + 7;

@@ -43,6 +43,15 @@ _SIGS = {
     "kz_sim_capture_steps": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_replay": (C.c_int, [C.c_void_p]),
     "kz_sim_query": (C.c_int, [C.c_void_p]),
+    "kz_group_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "kz_group_destroy": (None, [C.c_void_p]),
+    "kz_group_set_runs": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
+    "kz_group_set_implicit_launch": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_group_step_many": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_group_capture_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "kz_group_replay": (C.c_int, [C.c_void_p]),
+    "kz_group_query": (C.c_int, [C.c_void_p]),
+    "kz_group_sync": (C.c_int, [C.c_void_p]),
     "kz_sim_set_implicit_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kz_sim_scalars": (C.c_int, [C.c_void_p, C.c_double, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_void_p,
@@ -294,6 +303,59 @@ class Sim:
         ms, n = C.c_double(0.0), C.c_uint64(0)
         _chk(lib().kz_sim_kernel_time(self.h, C.byref(ms), C.byref(n)), "kz_sim_kernel_time")
         return ms.value, n.value
+
+
+class Group:
+    """Resident runs stepped together (include/kz_step.h kz_group_*): both
+    implicit passes of every step solve the cells of all member runs in one
+    set of launches; each run's other kernels stay on its own stream.  The
+    members end every step bit for bit where Sim.step_many leaves them."""
+
+    def __init__(self, sims=()):
+        h = C.c_void_p()
+        _chk(lib().kz_group_create(C.byref(h)), "kz_group_create")
+        self.h = h
+        self.sims = []
+        if sims:
+            self.set_runs(sims)
+
+    def set_runs(self, sims):
+        """New membership (waits for the previous members' work)."""
+        sims = list(sims)
+        arr = (C.c_void_p * max(len(sims), 1))(*[s.h for s in sims])
+        _chk(lib().kz_group_set_runs(self.h, arr, len(sims)), "kz_group_set_runs")
+        self.sims = sims
+
+    def set_implicit_launch(self, mode):
+        _chk(lib().kz_group_set_implicit_launch(self.h, int(mode)), "kz_group_set_implicit_launch")
+
+    def step_many(self, nsteps):
+        _chk(lib().kz_group_step_many(self.h, int(nsteps)), "kz_group_step_many")
+
+    def capture_steps(self, nsteps):
+        _chk(lib().kz_group_capture_steps(self.h, int(nsteps)), "kz_group_capture_steps")
+
+    def replay(self):
+        _chk(lib().kz_group_replay(self.h), "kz_group_replay")
+
+    def busy(self):
+        rc = lib().kz_group_query(self.h)
+        if rc < 0:
+            _chk(rc, "kz_group_query")
+        return rc == 1
+
+    def sync(self):
+        _chk(lib().kz_group_sync(self.h), "kz_group_sync")
+
+    def close(self):
+        if self.h:
+            lib().kz_group_destroy(self.h)
+            self.h = None
+            self.sims = []
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            self.close()
 
 
 class ImplicitBatch:

@@ -133,12 +133,21 @@ class _Resident:
 class RunServer:
     """Serve `specs` to completion (see the module docstring).  `hbm_budget`
     bounds the summed device bytes of the resident runs; a run that would
-    exceed it waits, except when nothing else is resident."""
+    exceed it waits, except when nothing else is resident.
 
-    def __init__(self, hbm_budget, batch_steps=10, log=None):
+    grouped=False: each run replays its own captured batch on its own
+    stream, as soon as its previous batch has finished.
+    grouped=True: the resident runs form one run group (hip.Group) and
+    advance together, batch by batch, by one captured graph of the whole
+    group; the implicit cells of all of them are solved by one set of
+    implicit launches per implicit pass.  Runs join and leave at batch
+    boundaries, where the group's graph is captured again."""
+
+    def __init__(self, hbm_budget, batch_steps=10, log=None, grouped=False):
         self.budget = int(hbm_budget)
         self.batch = int(batch_steps)
         self.log = log or (lambda *a: None)
+        self.grouped = bool(grouped)
 
     def serve(self, specs, prepared, on_finish=None):
         """Returns the wall time, zone-cycles, batch latencies per step (ms),
@@ -147,6 +156,8 @@ class RunServer:
         for s in specs:
             if s.nsteps % self.batch:
                 raise ValueError(f"run {s.rid}: {s.nsteps} steps is not a multiple of the batch {self.batch}")
+        if self.grouped:
+            return self._serve_grouped(specs, prepared, on_finish)
         need = footprints(specs, prepared)
         arrivals = collections.deque(sorted(specs, key=lambda s: s.arrival))
         lock = threading.Lock()
@@ -252,6 +263,138 @@ class RunServer:
             raise state["error"]
         return {"wall_s": wall, "zone_cycles": zc, "latency_ms": np.asarray(lat),
                 "peak_resident": peak_n, "peak_bytes": peak_b, "runs": n_done}
+
+
+    def _serve_grouped(self, specs, prepared, on_finish):
+        need = footprints(specs, prepared)
+        arrivals = collections.deque(sorted(specs, key=lambda s: s.arrival))
+        lock = threading.Lock()
+        # run creation and init (admission thread) never overlap a group
+        # capture or a device-wide free (stepping thread)
+        hip_lock = threading.Lock()
+        ready = collections.deque()
+        state = {"used": 0, "live": 0, "error": None, "stop": False}
+        freed = threading.Condition(lock)
+        t0 = time.perf_counter()
+
+        def admit():
+            try:
+                while arrivals and not state["stop"]:
+                    s = arrivals[0]
+                    delay = s.arrival - (time.perf_counter() - t0)
+                    if delay > 0:
+                        with freed:
+                            if not state["stop"]:
+                                freed.wait(delay)
+                        continue
+                    b = need[(s.nx, s.ny)]
+                    with freed:
+                        while state["live"] and state["used"] + b > self.budget and not state["stop"]:
+                            freed.wait(0.05)
+                        if state["stop"]:
+                            return
+                        state["used"] += b
+                        state["live"] += 1
+                    arrivals.popleft()
+                    with hip_lock:
+                        sim = start_run(s, prepared.pop(s.rid))
+                    with lock:
+                        ready.append(_Resident(s, sim, b))
+            except BaseException as e:  # surfaced by the stepping loop
+                with lock:
+                    state["error"] = e
+                    freed.notify_all()
+
+        th = threading.Thread(target=admit, daemon=True)
+        th.start()
+        grp = hip.Group()
+        members = []
+        launched = None
+        lat = []
+        zc = 0
+        peak_n = peak_b = 0
+        n_done = 0
+        regroups = 0
+        last_log = t0
+        try:
+            while True:
+                with lock:
+                    if state["error"] is not None:
+                        break
+                changed = False
+                if launched is not None:
+                    if grp.busy():
+                        time.sleep(1e-4)
+                        continue
+                    now = time.perf_counter()
+                    finished = []
+                    for r in members:
+                        lat.append((now - launched) * 1e3 / self.batch)
+                        r.done += self.batch
+                        zc += r.zones * self.batch
+                        if r.done >= r.spec.nsteps:
+                            finished.append(r)
+                    launched = None
+                    if finished:
+                        members = [r for r in members if r not in finished]
+                        with hip_lock:
+                            grp.set_runs([])  # waits for the batch; the group no longer refers to them
+                            for r in finished:
+                                r.sim.sync()  # raises on a device-side error of the run
+                                if on_finish is not None:
+                                    on_finish(r.spec, r.sim)
+                                r.sim.close()
+                        n_done += len(finished)
+                        with freed:
+                            for r in finished:
+                                state["used"] -= r.bytes
+                                state["live"] -= 1
+                            freed.notify_all()
+                        changed = True
+                with lock:
+                    joining = list(ready)
+                    ready.clear()
+                if joining:
+                    members.extend(joining)
+                    changed = True
+                peak_n = max(peak_n, len(members))
+                with lock:
+                    peak_b = max(peak_b, state["used"])
+                if not members:
+                    if not th.is_alive():
+                        with lock:
+                            if not ready:
+                                break
+                    time.sleep(1e-4)
+                    continue
+                if changed:
+                    with hip_lock:
+                        grp.set_runs([r.sim for r in members])
+                        grp.capture_steps(self.batch)
+                    regroups += 1
+                grp.replay()
+                launched = time.perf_counter()
+                now = launched
+                if now - last_log > 5.0:
+                    last_log = now
+                    self.log(f"  t={now - t0:.1f}s resident={len(members)} done={n_done}/{len(specs)} "
+                             f"zone-cycles={zc:.3e} regroups={regroups}")
+            wall = time.perf_counter() - t0
+        finally:
+            with freed:
+                state["stop"] = True
+                freed.notify_all()
+            th.join()
+            with lock:
+                members.extend(ready)
+            with hip_lock:
+                grp.close()  # waits for its stream
+                for r in members:
+                    r.sim.close()
+        if state["error"] is not None:
+            raise state["error"]
+        return {"wall_s": wall, "zone_cycles": zc, "latency_ms": np.asarray(lat),
+                "peak_resident": peak_n, "peak_bytes": peak_b, "runs": n_done, "regroups": regroups}
 
 
 def serve_sequential(specs, prepared, log=None, on_finish=None):
