@@ -44,17 +44,17 @@ BYTES_PER_UNIT = {"face_flux": 400.0, "update": 1224.0, "u2p": 504.0}
 # with a counting double (oracle/flops: +1 per + - * / sqrt, +2 per fused
 # multiply-add, transcendentals counted through their own operations).  `python oracle/flops/count.py c1`
 # over the 136 704 implicit cells of the 384x360 step after its first step:
-# 28 291.3 per solve on average; least-squares fit over the cells
-# flops = 14 565.3 + 9 985.4 * Newton iterations (rms 1.7e3).  The bench
+# 28 135.0 per solve on average; least-squares fit over the cells
+# flops = 14 588.6 + 9 854.7 * Newton iterations (rms 1.8e3).  The bench
 # multiplies the fit by the solves and iterations the device counts.
-IMPLICIT_FLOPS_PER_SOLVE = 14565.3
-IMPLICIT_FLOPS_PER_ITER = 9985.4
+IMPLICIT_FLOPS_PER_SOLVE = 14588.6
+IMPLICIT_FLOPS_PER_ITER = 9854.7
 # The independent-cell batch spends most of its operations in the later
 # rungs and in failing ladders, which the fit above does not describe:
 # `python oracle/flops/count.py c2` over 200 000 cells of the same generator
-# counts 134 630.8 operations per cell (first-rung solves 55 273.7, later
-# rungs 686 520.3, failed cells 2 625 525.3).
-BATCH_FLOPS_PER_CELL = 134630.8
+# counts 134 811.4 operations per cell (first-rung solves 55 463.7, later
+# rungs 686 479.6, failed cells 2 631 508.7).
+BATCH_FLOPS_PER_CELL = 134811.4
 # HBM bytes per launch of a kernel class at 384x360, from rocprofv3 runs of
 # FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
 # counter reads for a copy kernel of known size in the same run (k_soa2aos:

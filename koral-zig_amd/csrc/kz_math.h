@@ -21,16 +21,16 @@
  *           of 2^k.
  *   log   — reduction to m in [sqrt(1/2), sqrt(2)), then
  *           log(m) = 2 atanh(s), s = (m-1)/(m+1), with s carried as a
- *           double-double (exact product by fused multiply-add), odd series
- *           in s up to s^23.  Result available as a double-double (used by
+ *           double-double (one division, the remainder by fused
+ *           multiply-add), odd series in s up to s^23.  Result available as a double-double (used by
  *           pow).
  *   pow   — exp(y log x) evaluated in double-double: the product y*log(x) is
  *           formed exactly (TwoProduct by fused multiply-add) and the low part enters
  *           exp as a first-order correction, so the error does not grow with
  *           |y log x|.
- *   cbrt  — exponent/3 bit-level seed, three Halley iterations, final
- *           Newton correction with the residual x - t^3 formed in
- *           double-double.
+ *   cbrt  — division-free: Newton iterations for m^(-1/3) from an
+ *           exponent-field seed, t = m y^2, and a final Newton correction
+ *           with the residual m - t^3 formed in double-double.
  *   log1p — Goldberg (1991): log1p(x) = log(u) * x / (u - 1), u = 1 + x.
  *
  * The includer must define KZM_QUAL (e.g. `static inline` for C, or
@@ -154,10 +154,12 @@ KZM_QUAL void kzm_log_dd(double x, double *hi_out, double *lo_out) {
   double f = m - 1.0; /* exact for m in [1/2, 2] */
   double uh, ul;
   kzm_fast_two_sum(2.0, f, &uh, &ul);
-  double sh = f / uh;
-  double ph, pl;
-  kzm_two_prod(sh, uh, &ph, &pl);
-  double sl = (((f - ph) - pl) - sh * ul) / uh;
+  /* s = f / u as sh + sl with one division: sh = f * (1/uh) is within an
+   * ulp or two of the quotient, and the remainder f - sh*uh is exact by
+   * fma, so sl carries what sh misses */
+  double inv = 1.0 / uh;
+  double sh = f * inv;
+  double sl = (KZM_FMA(-sh, uh, f) - sh * ul) * inv;
   double s2 = sh * sh;
   /* tail = 2 s^3 (1/3 + s^2/5 + ... + s^20/23) */
   double p = 0x1.642c8590b2164p-4;                /* 2/23 */
@@ -274,20 +276,24 @@ KZM_QUAL double kzm_cbrt(double x) {
   int j = (et >= 0) ? et / 3 : -((-et + 2) / 3);
   int r3 = et - 3 * j; /* 0, 1, 2 */
   double m = kzm_from_bits((b & 0x000fffffffffffffULL) | ((uint64_t)(1023 + r3) << 52));
-  /* seed: bits(m)/3 + (2/3)*1023*2^52, ~6% accurate */
-  double t = kzm_from_bits(kzm_bits(m) / 3 + 0x2AA0000000000000ULL);
-  /* three Halley steps: t <- t (t^3 + 2m) / (2 t^3 + m) */
-  for (int i = 0; i < 3; ++i) {
-    double t3 = t * t * t;
-    t = t * (t3 + 2.0 * m) / (2.0 * t3 + m);
+  /* y ~ m^(-1/3) without division: seed from the exponent field,
+   * bits(y) ~ (4/3)*1023*2^52 - bits(m)/3 (under 10% off), then five Newton
+   * steps y <- y + y (1 - m y^3) / 3 (the error e goes to about 2 e^2) */
+  double y = kzm_from_bits(0x5540000000000000ULL - kzm_bits(m) / 3);
+  for (int i = 0; i < 5; ++i) {
+    double y3 = (y * y) * y;
+    y = KZM_FMA(y * KZM_FMA(-m, y3, 1.0), 0x1.5555555555555p-2, y);
   }
-  /* final Newton correction with an accurate residual m - t^3 */
+  double y2 = y * y; /* ~ 1 / cbrt(m)^2 */
+  double t = m * y2;
+  /* final Newton correction t + (m - t^3) / (3 t^2), the residual formed in
+   * double-double and 1 / t^2 taken as y^2 */
   double p2h, p2l, p3h, p3l;
   kzm_two_prod(t, t, &p2h, &p2l);
   kzm_two_prod(p2h, t, &p3h, &p3l);
   p3l += p2l * t;
   double resid = (m - p3h) - p3l;
-  t = t + resid / (3.0 * (t * t));
+  t = KZM_FMA(resid, y2 * 0x1.5555555555555p-2, t);
   return sign * (t * kzm_pow2i(j));
 }
 
