@@ -151,3 +151,32 @@ def test_grouped_server_matches_sequential():
 
     seq = server.serve_sequential(specs, prep2, on_finish=check)
     assert seq["zone_cycles"] == res["zone_cycles"]
+
+
+def test_group_full_physics_matches_oracle():
+    """Radiative viscosity and the dynamo on (BL sidecar loaded): the
+    viscous pass in each member's step head and the dynamo in its middle and
+    tail run on the member's stream around the group's implicit passes."""
+    pairs = []
+    for nx, ny, seed in ((24, 20, 51), (16, 12, 52)):
+        grid = T.grid_puffy(nx, ny)
+        phys = T.physics_puffy(radviscosity=True, dynamo=True)
+        tab = host.metric_tables(grid)
+        bl = host.bl_tables(grid)
+        cpu = kzo.Sim(grid, phys, tab)
+        gpu = hip.Sim(grid, phys, tab)
+        for s in (cpu, gpu):
+            s.set_bl_tables(bl)
+            host.init_puffy(s, grid, tab, phys, seed=seed)
+        pairs.append((cpu, gpu))
+    grp = hip.Group([g for _, g in pairs])
+    grp.capture_steps(1)
+    for rep in range(2):
+        grp.replay()
+        grp.sync()
+        for cpu, _ in pairs:
+            cpu.step()
+        check_all(pairs, f"replay {rep}")
+    grp.close()
+    for _, g in pairs:
+        g.close()
