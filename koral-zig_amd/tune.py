@@ -13,8 +13,10 @@ default unless another is at least 1 % faster).
 
     python koral-zig_amd/tune.py [--jobs N] [--nx 384 --ny 360] [--out DIR]
 
-from the repository root, on a machine with the GPU.  The defaults in
-kz_step.hip are the bounds this tuner selected.
+from the repository root, on a machine with the GPU.  The two halves can
+also run apart: `--build-only` compiles the libraries into --out (on any
+machine with hipcc), `--prebuilt` measures the libraries found there.  The
+defaults in kz_step.hip are the bounds this tuner selected.
 """
 import argparse
 import concurrent.futures as cf
@@ -109,6 +111,8 @@ def main():
     ap.add_argument("--ny", type=int, default=360)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--out", default=str(HERE / "lib" / "tune"))
+    ap.add_argument("--build-only", action="store_true", help="compile the variants into --out and stop")
+    ap.add_argument("--prebuilt", action="store_true", help="measure the variants already compiled in --out")
     ap.add_argument("--measure", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.measure:
@@ -118,13 +122,21 @@ def main():
     vs = variants()
     t0 = time.perf_counter()
     libs = {}
-    with cf.ThreadPoolExecutor(a.jobs) as ex:
-        futs = {ex.submit(build_variant, n, d, a.out): n for n, d in vs}
-        for fu in cf.as_completed(futs):
-            libs[futs[fu]] = fu.result()
-            print(f"  built {futs[fu]} ({time.perf_counter() - t0:.0f}s)", flush=True)
-    libs = {n: libs[n] for n, _ in vs}
-    print(f"built {len(libs)} variants in {time.perf_counter() - t0:.0f}s", flush=True)
+    if a.prebuilt:
+        for n, _ in vs:
+            libs[n] = Path(a.out) / ("libkz_hip_" + n.replace("=", "_") + ".so")
+            if not libs[n].exists():
+                raise RuntimeError(f"{libs[n]} missing: run with --build-only first")
+    else:
+        with cf.ThreadPoolExecutor(a.jobs) as ex:
+            futs = {ex.submit(build_variant, n, d, a.out): n for n, d in vs}
+            for fu in cf.as_completed(futs):
+                libs[futs[fu]] = fu.result()
+                print(f"  built {futs[fu]} ({time.perf_counter() - t0:.0f}s)", flush=True)
+        libs = {n: libs[n] for n, _ in vs}
+        print(f"built {len(libs)} variants in {time.perf_counter() - t0:.0f}s", flush=True)
+        if a.build_only:
+            return
     res = {}
     for name, lib in libs.items():
         r = subprocess.run([sys.executable, str(Path(__file__)), "--measure", str(lib), "--nx", str(a.nx), "--ny",
