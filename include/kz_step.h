@@ -45,6 +45,10 @@ int kz_sim_get_state(kz_sim *s, double *p, double *u, int32_t *flags);
 
 /* Sim.initCell's p2u over the domain (koral/sim.zig:672-685). */
 int kz_sim_p2u_domain(kz_sim *s);
+/* ct.calcDivB (koral/sim/ct.zig:336-381) at every domain cell of the current
+ * primitives: corner divergence of gdet*B over gdet, out[nz][ny][nx].
+ * Reads one ghost layer below each active dimension. Synchronous. */
+int kz_sim_calc_divb(kz_sim *s, double *out);
 /* Sim.setBc (koral/sim.zig:735-745 -> sim/bc.zig:40-73, PUFFY Bc.calc
  * koral/problems/puffy/puffy.zig:1080-1194, fillCorners2d bc.zig:286-377). */
 int kz_sim_set_bc(kz_sim *s, int ifinit);

@@ -28,6 +28,7 @@ _SIGS = {
     "kz_sim_set_state": (C.c_int, [C.c_void_p, D, D]),
     "kz_sim_get_state": (C.c_int, [C.c_void_p, D, D, I32]),
     "kz_sim_p2u_domain": (C.c_int, [C.c_void_p]),
+    "kz_sim_calc_divb": (C.c_int, [C.c_void_p, D]),
     "kz_sim_set_bc": (C.c_int, [C.c_void_p, C.c_int]),
     "kz_sim_postinit": (C.c_int, [C.c_void_p, C.c_double, D]),
     "kz_sim_init_timestep_guess": (C.c_int, [C.c_void_p]),
@@ -185,6 +186,13 @@ class Sim:
 
     def p2u_domain(self):
         _chk(lib().kz_sim_p2u_domain(self.h), "kz_sim_p2u_domain")
+
+    def calc_divb(self):
+        """ct.zig:336-381 calcDivB at every domain cell, [nz][ny][nx]."""
+        g = self.grid
+        out = np.zeros((g.nz, g.ny, g.nx))
+        _chk(lib().kz_sim_calc_divb(self.h, T.ptr(out)), "kz_sim_calc_divb")
+        return out
 
     def set_bc(self, ifinit=False):
         _chk(lib().kz_sim_set_bc(self.h, int(ifinit)), "kz_sim_set_bc")

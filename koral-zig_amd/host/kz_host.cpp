@@ -851,46 +851,54 @@ long kz_host_face_count(const kz_grid *gr, int dim) { return Grid(*gr).nface(dim
 // MetricCache.init (precompute.zig:175-213, fillCenters/fillFaces):
 // g, gcon: ncell*20; kris: ncell*64; gb[d], gconb[d]: nface(d)*20.
 int kz_host_metric_tables(const kz_grid *gr, double a, double mksr0, double mksh0, double *g, double *gcon,
-                          double *kris, double *gb0, double *gconb0, double *gb1, double *gconb1, double *gb2,
-                          double *gconb2) {
+                        double *kris, double *gb0, double *gconb0, double *gb1, double *gconb1, double *gb2,
+                        double *gconb2) {
   Grid G(*gr);
   MP mp{a, mksr0, mksh0};
-  for (long iz = -G.ngz; iz < gr->nz + G.ngz; ++iz)
-    for (long iy = -G.ngy; iy < gr->ny + G.ngy; ++iy)
-      for (long ix = -G.ngx; ix < gr->nx + G.ngx; ++ix) {
-        double x[4] = {0, G.xc(ix), G.yc(iy), G.zc(iz)};
-        CoordData d = compute(C_MKS2, mp, x);
-        apply_kris_correction(C_MKS2, mp, G, d, ix, iy, iz);
-        long ci = G.cidx(ix, iy, iz);
-        store_blocks(g + ci * 20, gcon + ci * 20, d);
-        for (int i = 0; i < 4; ++i)
-          for (int j = 0; j < 4; ++j)
-            for (int k = 0; k < 4; ++k) kris[ci * 64 + i * 16 + j * 4 + k] = d.kris[i][j][k];
-      }
+  // every point is independent: rows of cells (and of faces below) run in
+  // parallel, each point evaluated exactly as a serial loop would
+  const long ncy = gr->ny + 2 * G.ngy, ncz = gr->nz + 2 * G.ngz;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long row = 0; row < ncy * ncz; ++row) {
+    const long iz = row / ncy - G.ngz, iy = row % ncy - G.ngy;
+    for (long ix = -G.ngx; ix < gr->nx + G.ngx; ++ix) {
+      double x[4] = {0, G.xc(ix), G.yc(iy), G.zc(iz)};
+      CoordData d = compute(C_MKS2, mp, x);
+      apply_kris_correction(C_MKS2, mp, G, d, ix, iy, iz);
+      long ci = G.cidx(ix, iy, iz);
+      store_blocks(g + ci * 20, gcon + ci * 20, d);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+          for (int k = 0; k < 4; ++k) kris[ci * 64 + i * 16 + j * 4 + k] = d.kris[i][j][k];
+    }
+  }
   double *gb[3] = {gb0, gb1, gb2}, *gcb[3] = {gconb0, gconb1, gconb2};
   long nxh = gr->nx + G.ngx, nyh = gr->ny + G.ngy, nzh = gr->nz + G.ngz;
-  for (long iz = -G.ngz; iz <= nzh; ++iz)
-    for (long iy = -G.ngy; iy <= nyh; ++iy)
-      for (long ix = -G.ngx; ix <= nxh; ++ix) {
-        if (iy < nyh && iz < nzh) {
-          double x[4] = {0, G.xl(ix), G.yc(iy), G.zc(iz)};
-          CoordData d = compute(C_MKS2, mp, x);
-          long fi = G.fidx(0, ix, iy, iz);
-          store_blocks(gb[0] + fi * 20, gcb[0] + fi * 20, d);
-        }
-        if (ix < nxh && iz < nzh) {
-          double x[4] = {0, G.xc(ix), G.yl(iy), G.zc(iz)};
-          CoordData d = compute(C_MKS2, mp, x);
-          long fi = G.fidx(1, ix, iy, iz);
-          store_blocks(gb[1] + fi * 20, gcb[1] + fi * 20, d);
-        }
-        if (ix < nxh && iy < nyh) {
-          double x[4] = {0, G.xc(ix), G.yc(iy), G.zl(iz)};
-          CoordData d = compute(C_MKS2, mp, x);
-          long fi = G.fidx(2, ix, iy, iz);
-          store_blocks(gb[2] + fi * 20, gcb[2] + fi * 20, d);
-        }
+  const long nfy = nyh + G.ngy + 1, nfz = nzh + G.ngz + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long row = 0; row < nfy * nfz; ++row) {
+    const long iz = row / nfy - G.ngz, iy = row % nfy - G.ngy;
+    for (long ix = -G.ngx; ix <= nxh; ++ix) {
+      if (iy < nyh && iz < nzh) {
+        double x[4] = {0, G.xl(ix), G.yc(iy), G.zc(iz)};
+        CoordData d = compute(C_MKS2, mp, x);
+        long fi = G.fidx(0, ix, iy, iz);
+        store_blocks(gb[0] + fi * 20, gcb[0] + fi * 20, d);
       }
+      if (ix < nxh && iz < nzh) {
+        double x[4] = {0, G.xc(ix), G.yl(iy), G.zc(iz)};
+        CoordData d = compute(C_MKS2, mp, x);
+        long fi = G.fidx(1, ix, iy, iz);
+        store_blocks(gb[1] + fi * 20, gcb[1] + fi * 20, d);
+      }
+      if (ix < nxh && iy < nyh) {
+        double x[4] = {0, G.xc(ix), G.yc(iy), G.zl(iz)};
+        CoordData d = compute(C_MKS2, mp, x);
+        long fi = G.fidx(2, ix, iy, iz);
+        store_blocks(gb[2] + fi * 20, gcb[2] + fi * 20, d);
+      }
+    }
+  }
   return 0;
 }
 

@@ -59,6 +59,8 @@ def lib():
         L.kzo_sim_get_state.restype = None
         L.kzo_sim_get_scal.argtypes = [vp, _D]
         L.kzo_sim_get_scal.restype = None
+        L.kzo_sim_calc_divb.argtypes = [vp, _D]
+        L.kzo_sim_calc_divb.restype = None
         L.kzo_sim_get_flb.argtypes = [vp, C.c_int, _D]
         L.kzo_sim_get_flb.restype = None
         L.kzo_sim_set_flb.argtypes = [vp, C.c_int, _D]
@@ -382,6 +384,13 @@ class Sim:
         lib().kzo_sim_get_scaleth(self.h, T.ptr(h))
         return h
 
+    def calc_divb(self):
+        """ct.zig:336-381 calcDivB at every domain cell, [nz][ny][nx]."""
+        g = self.grid
+        out = np.zeros((g.nz, g.ny, g.nx))
+        lib().kzo_sim_calc_divb(self.h, T.ptr(out))
+        return out
+
     def get_scal(self):
         s = np.zeros(self.shape + (20,))
         lib().kzo_sim_get_scal(self.h, T.ptr(s))
@@ -453,6 +462,27 @@ def true_residual(pp, pp00, dt, geom, gam, op, wp, we, wf):
                                   C.c_double(gam), C.byref(op), T.ptr(wp, C.c_int32), T.ptr(we, C.c_int32),
                                   T.ptr(wf, C.c_int32), T.ptr(err))
     return err
+
+
+WP_RAD, WP_MHD = 1, 2
+WE_ENERGY, WE_ENTROPY = 0, 1
+WF_LAB, WF_FF = 0, 1
+
+
+def solve_4d_prim(pp00, geom, dt, gam, rp, op, ip, wp, we=WE_ENERGY, wf=WF_LAB):
+    """One rung of the implicit solve from pp00 (implicit.zig solve4dPrim,
+    no warm start, no ladder): (ok, iters, pp, internal uu)."""
+    n = len(pp00)
+    pp00 = np.ascontiguousarray(pp00, np.float64)
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+    pp = pp00.copy()
+    uu = np.zeros_like(pp00)
+    ok = np.zeros(n, np.int32)
+    it = np.zeros(n, np.int32)
+    lib().kzo_solve_4d_prim_batch(C.c_int64(n), T.ptr(pp00), T.ptr(_geoms(geom)), T.ptr(dt), C.c_double(gam),
+                                  C.byref(rp), C.byref(op), C.byref(ip), C.c_int32(wp), C.c_int32(we),
+                                  C.c_int32(wf), T.ptr(pp), T.ptr(uu), T.ptr(ok, C.c_int32), T.ptr(it, C.c_int32))
+    return ok, it, pp, uu
 
 
 def step_function(x, x9):

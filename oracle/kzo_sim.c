@@ -1334,6 +1334,47 @@ static int fluxes_at_faces(kzo_sim *s) {
   return KZO_OK;
 }
 
+/* ct.zig:336-381 calcDivB: corner divergence of gdet*B from cell-centred
+ * values, over every domain cell into out[(iz*ny + iy)*nx + ix]; reads one
+ * ghost layer below each active dimension. */
+static double gdet_pb(const kzo_sim *s, int iv, long ix, long iy, long iz) {
+  long ci = cidx(s, ix, iy, iz);
+  return s->g[ci * 20 + 19] * s->p[ci * NV + iv];
+}
+void kzo_sim_calc_divb(kzo_sim *s, double *out) {
+  const long nx = s->grid.nx, ny = s->grid.ny, nz = s->grid.nz;
+  for (long iz = 0; iz < nz; ++iz)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long ix = 0; ix < nx; ++ix) {
+        double divb;
+        if (nz == 1) {
+          divb = (gdet_pb(s, B1, ix, iy, iz) + gdet_pb(s, B1, ix, iy - 1, iz) - gdet_pb(s, B1, ix - 1, iy, iz) -
+                  gdet_pb(s, B1, ix - 1, iy - 1, iz)) /
+                     (2.0 * (xc_(s, ix + 1) - xc_(s, ix))) +
+                 (gdet_pb(s, B2, ix, iy, iz) + gdet_pb(s, B2, ix - 1, iy, iz) - gdet_pb(s, B2, ix, iy - 1, iz) -
+                  gdet_pb(s, B2, ix - 1, iy - 1, iz)) /
+                     (2.0 * (yc_(s, iy + 1) - yc_(s, iy)));
+        } else {
+          divb = (gdet_pb(s, B1, ix, iy, iz) + gdet_pb(s, B1, ix, iy - 1, iz) - gdet_pb(s, B1, ix - 1, iy, iz) -
+                  gdet_pb(s, B1, ix - 1, iy - 1, iz) + gdet_pb(s, B1, ix, iy, iz - 1) +
+                  gdet_pb(s, B1, ix, iy - 1, iz - 1) - gdet_pb(s, B1, ix - 1, iy, iz - 1) -
+                  gdet_pb(s, B1, ix - 1, iy - 1, iz - 1)) /
+                     (4.0 * (xc_(s, ix) - xc_(s, ix - 1))) +
+                 (gdet_pb(s, B2, ix, iy, iz) + gdet_pb(s, B2, ix - 1, iy, iz) - gdet_pb(s, B2, ix, iy - 1, iz) -
+                  gdet_pb(s, B2, ix - 1, iy - 1, iz) + gdet_pb(s, B2, ix, iy, iz - 1) +
+                  gdet_pb(s, B2, ix - 1, iy, iz - 1) - gdet_pb(s, B2, ix, iy - 1, iz - 1) -
+                  gdet_pb(s, B2, ix - 1, iy - 1, iz - 1)) /
+                     (4.0 * (yc_(s, iy) - yc_(s, iy - 1))) +
+                 (gdet_pb(s, B3, ix, iy, iz) + gdet_pb(s, B3, ix - 1, iy, iz) - gdet_pb(s, B3, ix, iy, iz - 1) -
+                  gdet_pb(s, B3, ix - 1, iy, iz - 1) + gdet_pb(s, B3, ix, iy - 1, iz) +
+                  gdet_pb(s, B3, ix - 1, iy - 1, iz) - gdet_pb(s, B3, ix, iy - 1, iz - 1) -
+                  gdet_pb(s, B3, ix - 1, iy - 1, iz - 1)) /
+                     (4.0 * (zc_(s, iz) - zc_(s, iz - 1)));
+        }
+        out[(iz * ny + iy) * nx + ix] = divb / s->g[cidx(s, ix, iy, iz) * 20 + 19];
+      }
+}
+
 /* ct.zig:38-177 fluxCt */
 static long emf_idx(const kzo_sim *s, long ix, long iy, long iz) {
   return (iz * (s->grid.ny + 1) + iy) * (long)(s->grid.nx + 1) + ix;
