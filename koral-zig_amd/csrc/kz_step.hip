@@ -870,6 +870,47 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, 
 
 
 // ---------------------------------------------------------------------------
+// calcDivB (ct.zig:336-381): corner divergence of gdet*B from the cell-centred
+// values of the cell and its lower neighbours (one ghost layer below each
+// active dimension), divided by the cell's gdet; one thread per domain cell,
+// out[ix + nx*(iy + ny*iz)].  The reference's 2D form divides by twice the
+// centre spacing above the cell, its 3D form by four times the spacing below.
+__device__ __forceinline__ double gdet_pb(const GridD &g, const Dev &d, int iv, long ix, long iy, long iz) {
+  long c = cidx(g, ix, iy, iz);
+  return d.gc[26 * g.ncell + c] * d.p[iv * g.ncell + c];
+}
+__global__ void __launch_bounds__(256) k_divb(GridD g, Dev d, double *out) {
+  long t = TID;
+  if (t >= (long)g.nx * g.ny * g.nz) return;
+  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
+  double divb;
+  if (g.nz == 1) {
+    divb = (gdet_pb(g, d, B1, ix, iy, iz) + gdet_pb(g, d, B1, ix, iy - 1, iz) - gdet_pb(g, d, B1, ix - 1, iy, iz) -
+            gdet_pb(g, d, B1, ix - 1, iy - 1, iz)) /
+               (2.0 * (xc_(g, ix + 1) - xc_(g, ix))) +
+           (gdet_pb(g, d, B2, ix, iy, iz) + gdet_pb(g, d, B2, ix - 1, iy, iz) - gdet_pb(g, d, B2, ix, iy - 1, iz) -
+            gdet_pb(g, d, B2, ix - 1, iy - 1, iz)) /
+               (2.0 * (yc_(g, iy + 1) - yc_(g, iy)));
+  } else {
+    double sx = gdet_pb(g, d, B1, ix, iy, iz) + gdet_pb(g, d, B1, ix, iy - 1, iz) - gdet_pb(g, d, B1, ix - 1, iy, iz) -
+                gdet_pb(g, d, B1, ix - 1, iy - 1, iz) + gdet_pb(g, d, B1, ix, iy, iz - 1) +
+                gdet_pb(g, d, B1, ix, iy - 1, iz - 1) - gdet_pb(g, d, B1, ix - 1, iy, iz - 1) -
+                gdet_pb(g, d, B1, ix - 1, iy - 1, iz - 1);
+    double sy = gdet_pb(g, d, B2, ix, iy, iz) + gdet_pb(g, d, B2, ix - 1, iy, iz) - gdet_pb(g, d, B2, ix, iy - 1, iz) -
+                gdet_pb(g, d, B2, ix - 1, iy - 1, iz) + gdet_pb(g, d, B2, ix, iy, iz - 1) +
+                gdet_pb(g, d, B2, ix - 1, iy, iz - 1) - gdet_pb(g, d, B2, ix, iy - 1, iz - 1) -
+                gdet_pb(g, d, B2, ix - 1, iy - 1, iz - 1);
+    double sz = gdet_pb(g, d, B3, ix, iy, iz) + gdet_pb(g, d, B3, ix - 1, iy, iz) - gdet_pb(g, d, B3, ix, iy, iz - 1) -
+                gdet_pb(g, d, B3, ix - 1, iy, iz - 1) + gdet_pb(g, d, B3, ix, iy - 1, iz) +
+                gdet_pb(g, d, B3, ix - 1, iy - 1, iz) - gdet_pb(g, d, B3, ix, iy - 1, iz - 1) -
+                gdet_pb(g, d, B3, ix - 1, iy - 1, iz - 1);
+    divb = sx / (4.0 * (xc_(g, ix) - xc_(g, ix - 1))) + sy / (4.0 * (yc_(g, iy) - yc_(g, iy - 1))) +
+           sz / (4.0 * (zc_(g, iz) - zc_(g, iz - 1)));
+  }
+  out[t] = divb / d.gc[26 * g.ncell + cidx(g, ix, iy, iz)];
+}
+
+// ---------------------------------------------------------------------------
 // flux-CT (ct.zig:38-177)
 __device__ __forceinline__ long pin(long n, long i) { return n == 1 ? 0 : i; }
 __device__ __forceinline__ double fb(const GridD &g, const Dev &d, int dim, int iv, long ix, long iy, long iz) {
@@ -2180,6 +2221,13 @@ struct TimedLaunch {
     }                                                                               \
   } while (0)
 
+// keep the first error of a sequence of set-up calls (error codes are
+// distinct negative values: OR-ing them would name another error)
+static inline void first_err(int &e, int code) {
+  if (!e) e = code;
+}
+static inline int hip_err(hipError_t r) { return r == hipSuccess ? 0 : KZ_E_HIP; }
+
 static int dalloc(kz_sim *s, void **p, size_t bytes) {
   if (hipMalloc(p, bytes) != hipSuccess) return KZ_E_OOM;
   s->allocs.push_back(*p);
@@ -2280,50 +2328,49 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   Dev &D = s->d;
   double **fl[] = {&D.p, &D.u, &D.ut0, &D.ut1, &D.ut2, &D.dut1, &D.dut2, &D.drt1, &D.drt2, &D.uforget, &D.stage};
   int e = 0;
-  for (auto f : fl) e |= dalloc(s, (void **)f, fb);
-  for (int d = 0; d < 3; ++d) e |= dalloc(s, (void **)&D.flb[d], (size_t)G.nface[d] * NV * sizeof(double));
-  e |= dalloc(s, (void **)&D.scal, (size_t)G.ncell * SC_N * sizeof(double));
-  e |= dalloc(s, (void **)&D.flags, (size_t)G.ncell * KZ_NFLAGS * sizeof(int32_t));
+  for (auto f : fl) first_err(e, dalloc(s, (void **)f, fb));
+  for (int d = 0; d < 3; ++d) first_err(e, dalloc(s, (void **)&D.flb[d], (size_t)G.nface[d] * NV * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.scal, (size_t)G.ncell * SC_N * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.flags, (size_t)G.ncell * KZ_NFLAGS * sizeof(int32_t)));
   long ncorn = (long)(gr->nx + 1) * (gr->ny + 1) * (gr->nz + 1);
-  for (int c = 0; c < 3; ++c) e |= dalloc(s, (void **)&D.emf[c], ncorn * sizeof(double));
-  e |= dalloc(s, (void **)&D.gc, (size_t)G.ncell * GEO_NC * sizeof(double));
-  for (int d = 0; d < 3; ++d) e |= dalloc(s, (void **)&D.gf[d], (size_t)G.nface[d] * GEO_NC * sizeof(double));
-  e |= dalloc(s, (void **)&D.kris, (size_t)G.ncell * KR_NC * sizeof(double));
+  for (int c = 0; c < 3; ++c) first_err(e, dalloc(s, (void **)&D.emf[c], ncorn * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.gc, (size_t)G.ncell * GEO_NC * sizeof(double)));
+  for (int d = 0; d < 3; ++d) first_err(e, dalloc(s, (void **)&D.gf[d], (size_t)G.nface[d] * GEO_NC * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.kris, (size_t)G.ncell * KR_NC * sizeof(double)));
   size_t nbcx = (size_t)gr->nz * gr->ny * gr->ng * BCX_REC;
-  e |= dalloc(s, (void **)&D.bcx, nbcx * sizeof(double));
-  e |= dalloc(s, (void **)&D.cnt, 8 * sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.tsd, 2 * sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.err, sizeof(int));
-  e |= dalloc(s, (void **)&D.ss, SS_N * sizeof(double));
+  first_err(e, dalloc(s, (void **)&D.bcx, nbcx * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.cnt, 8 * sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.tsd, 2 * sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.err, sizeof(int)));
+  first_err(e, dalloc(s, (void **)&D.ss, SS_N * sizeof(double)));
   long nd = (long)gr->nx * gr->ny * gr->nz;
-  e |= dalloc(s, (void **)&D.imp_uu00, (size_t)nd * NV * sizeof(double));
-  e |= dalloc(s, (void **)&D.imp_pp0, (size_t)nd * NV * sizeof(double));
-  e |= dalloc(s, (void **)&D.imp_aux, (size_t)nd * IMP_NAUX * sizeof(double));
-  e |= dalloc(s, (void **)&D.imp_slow, (size_t)nd * sizeof(int32_t));
-  e |= dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.imp_list, (size_t)2 * nd * sizeof(int32_t));
-  e |= dalloc(s, (void **)&D.imp_ncnt, 2 * sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.imp_rl, (size_t)2 * nd * sizeof(int32_t));
-  e |= dalloc(s, (void **)&D.imp_rc, 2 * sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.imp_qpos, 3 * sizeof(unsigned long long));
-  e |= dalloc(s, (void **)&D.sync, SY_N * sizeof(unsigned));
-  e |= dalloc(s, (void **)&D.fix_list, (size_t)nd * sizeof(int32_t));
-  if (ph->radviscosity || ph->dynamo) e |= dalloc(s, (void **)&D.bl, (size_t)G.ncell * BL_NC * sizeof(double));
-  if (ph->radviscosity) e |= dalloc(s, (void **)&D.rijvisc, (size_t)G.ncell * 16 * sizeof(double));
+  first_err(e, dalloc(s, (void **)&D.imp_uu00, (size_t)nd * NV * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.imp_pp0, (size_t)nd * NV * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.imp_aux, (size_t)nd * IMP_NAUX * sizeof(double)));
+  first_err(e, dalloc(s, (void **)&D.imp_slow, (size_t)nd * sizeof(int32_t)));
+  first_err(e, dalloc(s, (void **)&D.imp_nslow, sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.imp_list, (size_t)2 * nd * sizeof(int32_t)));
+  first_err(e, dalloc(s, (void **)&D.imp_ncnt, 2 * sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.imp_rl, (size_t)2 * nd * sizeof(int32_t)));
+  first_err(e, dalloc(s, (void **)&D.imp_rc, 2 * sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.imp_qpos, 3 * sizeof(unsigned long long)));
+  first_err(e, dalloc(s, (void **)&D.sync, SY_N * sizeof(unsigned)));
+  first_err(e, dalloc(s, (void **)&D.fix_list, (size_t)nd * sizeof(int32_t)));
+  if (ph->radviscosity || ph->dynamo) first_err(e, dalloc(s, (void **)&D.bl, (size_t)G.ncell * BL_NC * sizeof(double)));
+  if (ph->radviscosity) first_err(e, dalloc(s, (void **)&D.rijvisc, (size_t)G.ncell * 16 * sizeof(double)));
   if (ph->dynamo) {
-    e |= dalloc(s, (void **)&D.dyn_a, (size_t)G.ncell * sizeof(double));
-    e |= dalloc(s, (void **)&D.scaleth, (size_t)gr->nx * sizeof(double));
+    first_err(e, dalloc(s, (void **)&D.dyn_a, (size_t)G.ncell * sizeof(double)));
+    first_err(e, dalloc(s, (void **)&D.scaleth, (size_t)gr->nx * sizeof(double)));
   }
   if (e) { *err = e < 0 ? e : KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // metric tables
-  e |= hipMemcpyAsync(D.gc, pk_gc.data(), pk_gc.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
+  first_err(e, hip_err(hipMemcpyAsync(D.gc, pk_gc.data(), pk_gc.size() * sizeof(double), hipMemcpyHostToDevice, s->stream)));
   for (int d = 0; d < 3; ++d)
-    e |= hipMemcpyAsync(D.gf[d], pk_gf[d].data(), pk_gf[d].size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-         hipSuccess;
-  e |= hipMemcpyAsync(D.kris, pk_kr.data(), pk_kr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
-  e |= hipMemcpyAsync(D.bcx, bcx, nbcx * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess;
+    first_err(e, hip_err(hipMemcpyAsync(D.gf[d], pk_gf[d].data(), pk_gf[d].size() * sizeof(double), hipMemcpyHostToDevice, s->stream)));
+  first_err(e, hip_err(hipMemcpyAsync(D.kris, pk_kr.data(), pk_kr.size() * sizeof(double), hipMemcpyHostToDevice, s->stream)));
+  first_err(e, hip_err(hipMemcpyAsync(D.bcx, bcx, nbcx * sizeof(double), hipMemcpyHostToDevice, s->stream)));
   // the host staging vectors must outlive the copies
-  e |= hipStreamSynchronize(s->stream) != hipSuccess;
+  first_err(e, hip_err(hipStreamSynchronize(s->stream)));
   if (e) { *err = KZ_E_HIP; kz_sim_destroy(s); return nullptr; }
   // set_grid minimum cell sizes incl. the mdx quirk (sim.zig:447-466)
   double mdx = -1, mdy = -1, mdz = -1;
@@ -2429,6 +2476,16 @@ static long ndom(const kz_sim *s) { return (long)s->g.nx * s->g.ny * s->g.nz; }
 int kz_sim_p2u_domain(kz_sim *s) {
   k_p2u_domain<<<nblk(ndom(s), 256), 256, 0, s->stream>>>(s->g, s->d, s->ph.gam);
   return check_err(s);
+}
+
+int kz_sim_calc_divb(kz_sim *s, double *out) {
+  if (!s || !out) return KZ_E_INVALID_CONFIG;
+  const long nd = ndom(s);
+  k_divb<<<nblk(nd, 256), 256, 0, s->stream>>>(s->g, s->d, s->d.stage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->d.stage, nd * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
 }
 
 // cells of the domain plus one ring in every active dimension
@@ -2869,16 +2926,16 @@ int kz_group_set_runs(kz_group *G, kz_sim *const *sims, int nsims) {
     group_free_work(G);
     const long c = n + n / 4;
     int e = 0;
-    e |= group_alloc(G, (void **)&G->w.uu00, (size_t)c * NV * sizeof(double));
-    e |= group_alloc(G, (void **)&G->w.pp0, (size_t)c * NV * sizeof(double));
-    e |= group_alloc(G, (void **)&G->w.aux, (size_t)c * IMP_NAUX * sizeof(double));
-    e |= group_alloc(G, (void **)&G->w.slow, (size_t)c * sizeof(int32_t));
-    e |= group_alloc(G, (void **)&G->w.list, (size_t)2 * c * sizeof(int32_t));
-    e |= group_alloc(G, (void **)&G->w.nslow, sizeof(unsigned long long));
-    e |= group_alloc(G, (void **)&G->w.ncnt, 2 * sizeof(unsigned long long));
-    e |= group_alloc(G, (void **)&G->w.rl, (size_t)2 * c * sizeof(int32_t));
-    e |= group_alloc(G, (void **)&G->w.rc, 2 * sizeof(unsigned long long));
-    e |= group_alloc(G, (void **)&G->w.qpos, 3 * sizeof(unsigned long long));
+    first_err(e, group_alloc(G, (void **)&G->w.uu00, (size_t)c * NV * sizeof(double)));
+    first_err(e, group_alloc(G, (void **)&G->w.pp0, (size_t)c * NV * sizeof(double)));
+    first_err(e, group_alloc(G, (void **)&G->w.aux, (size_t)c * IMP_NAUX * sizeof(double)));
+    first_err(e, group_alloc(G, (void **)&G->w.slow, (size_t)c * sizeof(int32_t)));
+    first_err(e, group_alloc(G, (void **)&G->w.list, (size_t)2 * c * sizeof(int32_t)));
+    first_err(e, group_alloc(G, (void **)&G->w.nslow, sizeof(unsigned long long)));
+    first_err(e, group_alloc(G, (void **)&G->w.ncnt, 2 * sizeof(unsigned long long)));
+    first_err(e, group_alloc(G, (void **)&G->w.rl, (size_t)2 * c * sizeof(int32_t)));
+    first_err(e, group_alloc(G, (void **)&G->w.rc, 2 * sizeof(unsigned long long)));
+    first_err(e, group_alloc(G, (void **)&G->w.qpos, 3 * sizeof(unsigned long long)));
     if (e) {
       group_free_work(G);
       return KZ_E_HIP;
@@ -3443,28 +3500,28 @@ kz_implicit_batch *kz_implicit_batch_create(int64_t n, const kz_physics *ph, int
     return nullptr;
   }
   int e = 0;
-  e |= balloc(b, (void **)&b->gp, (size_t)n * (GEO_NC + 1) * sizeof(double));
-  e |= balloc(b, (void **)&b->dt, (size_t)n * sizeof(double));
-  for (double **f : {&b->uu, &b->pp, &b->pp_in, &b->stage}) e |= balloc(b, (void **)f, (size_t)n * NV * sizeof(double));
-  e |= balloc(b, (void **)&b->res, (size_t)n * sizeof(kz_implicit_result));
-  e |= balloc(b, (void **)&b->cnt, 4 * sizeof(unsigned long long));
-  e |= balloc(b, (void **)&b->w.uu00, (size_t)n * NV * sizeof(double));
-  e |= balloc(b, (void **)&b->w.pp0, (size_t)n * NV * sizeof(double));
-  e |= balloc(b, (void **)&b->w.aux, (size_t)n * IMP_NAUX * sizeof(double));
-  e |= balloc(b, (void **)&b->w.slow, (size_t)n * sizeof(int32_t));
-  e |= balloc(b, (void **)&b->w.list, (size_t)2 * n * sizeof(int32_t));
-  e |= balloc(b, (void **)&b->w.nslow, sizeof(unsigned long long));
-  e |= balloc(b, (void **)&b->w.ncnt, 2 * sizeof(unsigned long long));
-  e |= balloc(b, (void **)&b->w.rl, (size_t)2 * n * sizeof(int32_t));
-  e |= balloc(b, (void **)&b->w.rc, 2 * sizeof(unsigned long long));
-  e |= balloc(b, (void **)&b->w.qpos, 3 * sizeof(unsigned long long));
+  first_err(e, balloc(b, (void **)&b->gp, (size_t)n * (GEO_NC + 1) * sizeof(double)));
+  first_err(e, balloc(b, (void **)&b->dt, (size_t)n * sizeof(double)));
+  for (double **f : {&b->uu, &b->pp, &b->pp_in, &b->stage}) first_err(e, balloc(b, (void **)f, (size_t)n * NV * sizeof(double)));
+  first_err(e, balloc(b, (void **)&b->res, (size_t)n * sizeof(kz_implicit_result)));
+  first_err(e, balloc(b, (void **)&b->cnt, 4 * sizeof(unsigned long long)));
+  first_err(e, balloc(b, (void **)&b->w.uu00, (size_t)n * NV * sizeof(double)));
+  first_err(e, balloc(b, (void **)&b->w.pp0, (size_t)n * NV * sizeof(double)));
+  first_err(e, balloc(b, (void **)&b->w.aux, (size_t)n * IMP_NAUX * sizeof(double)));
+  first_err(e, balloc(b, (void **)&b->w.slow, (size_t)n * sizeof(int32_t)));
+  first_err(e, balloc(b, (void **)&b->w.list, (size_t)2 * n * sizeof(int32_t)));
+  first_err(e, balloc(b, (void **)&b->w.nslow, sizeof(unsigned long long)));
+  first_err(e, balloc(b, (void **)&b->w.ncnt, 2 * sizeof(unsigned long long)));
+  first_err(e, balloc(b, (void **)&b->w.rl, (size_t)2 * n * sizeof(int32_t)));
+  first_err(e, balloc(b, (void **)&b->w.rc, 2 * sizeof(unsigned long long)));
+  first_err(e, balloc(b, (void **)&b->w.qpos, 3 * sizeof(unsigned long long)));
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      e |= KZ_E_HIP;
+      first_err(e, KZ_E_HIP);
     b->persist_blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);  // one wave per SIMD
   }
-  for (auto &x : b->ev) e |= hipEventCreate(&x) != hipSuccess;
+  for (auto &x : b->ev) first_err(e, hip_err(hipEventCreate(&x)));
   if (e) {
     *err = e < 0 ? e : KZ_E_HIP;
     kz_implicit_batch_destroy(b);
@@ -3553,3 +3610,4 @@ int kz_implicit_batch_get(kz_implicit_batch *b, double *uu, double *pp, kz_impli
 }
 
 }  // extern "C"
+
