@@ -861,7 +861,7 @@ static void fill_state(const double *pp, const impl_ctx *cx, const kz_opac_param
  * 4-lane batch is bit-identical to it, simd_tests.zig). Returns ok; iters. */
 static int solve_4d_prim_core(const double *uu00, const kzo_radstate *state00, const double *pp0in, double dt,
                               const impl_ctx *cx, const kz_opac_params *op, int whichprim, int whicheq,
-                              int whichframe, double *pp_out, int *iters_out) {
+                              int whichframe, double *pp_out, int *iters_out, double *uu_out) {
   const kz_implicit_params *ip = cx->ip;
   const kzo_geo *gm = cx->gm;
   double uu[NV], pp[NV], ppp[NV];
@@ -1006,6 +1006,8 @@ static int solve_4d_prim_core(const double *uu00, const kzo_radstate *state00, c
   }
   if (iter > ip->maxiter || failed) return 0;
   for (int i = 0; i < NV; ++i) pp_out[i] = pp[i];
+  if (uu_out)
+    for (int i = 0; i < NV; ++i) uu_out[i] = uu[i];
   *iters_out = iter;
   return 1;
 }
@@ -1042,7 +1044,7 @@ kz_implicit_result kzo_solve_implicit_lab(double *uu, double *pp, const kzo_geo 
       double ppw[NV];
       for (int i = 0; i < NV; ++i) ppw[i] = pp[i];
       int iters = 0;
-      int ok = solve_4d_prim_core(uu00, &state00, pp0, dt, &cx, &opl, rp_[ir], re_[ir], rf_[ir], ppw, &iters);
+      int ok = solve_4d_prim_core(uu00, &state00, pp0, dt, &cx, &opl, rp_[ir], re_[ir], rf_[ir], ppw, &iters, 0);
       if (ok) {
         for (int i = 0; i < NV; ++i) pp[i] = ppw[i];
         kzo_p2u(ppw, uu, gm, gam);
@@ -1052,6 +1054,32 @@ kz_implicit_result kzo_solve_implicit_lab(double *uu, double *pp, const kzo_geo 
     }
   }
   return fail;
+}
+
+/* implicit.zig:924-926 solve4dPrim called on its own (no bisection warm
+ * start, no ladder), as the reference's implicit theory tests call it
+ * (koral/tests/implicit_tests.zig:89-130, 305-340): one rung from pp00 with
+ * the given primitive block, equation and frame.  uu_out gets the solver's
+ * internal conserved vector (the constraint construction), pp_out the
+ * primitives; ok[i] 0 leaves both untouched. */
+int kzo_solve_4d_prim_batch(int64_t n, const double *pp00, const kz_geom *g, const double *dt, double gam,
+                            const kz_rad_params *rp, const kz_opac_params *op, const kz_implicit_params *ip,
+                            int32_t whichprim, int32_t whicheq, int32_t whichframe, double *pp_out, double *uu_out,
+                            int32_t *ok, int32_t *iters) {
+  for (int64_t i = 0; i < n; ++i) {
+    kzo_geo gm;
+    kzo_geo_from_kz(&g[i], &gm);
+    impl_ctx cx = {&gm, gam, rp, op, ip};
+    double uu00[NV];
+    kzo_p2u(pp00 + i * NV, uu00, &gm, gam);
+    kzo_radstate st00;
+    kzo_fill_radstate(pp00 + i * NV, &gm, gam, op, 1, &st00);
+    int it = 0;
+    ok[i] = solve_4d_prim_core(uu00, &st00, pp00 + i * NV, dt[i], &cx, op, whichprim, whicheq, whichframe,
+                               pp_out + i * NV, &it, uu_out + i * NV);
+    iters[i] = it;
+  }
+  return 0;
 }
 
 /* implicit_golden_tests.zig:25-40 trueResidual: the max normalized error of
