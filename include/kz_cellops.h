@@ -19,7 +19,10 @@
 extern "C" {
 #endif
 
-/* p2u (koral/p2u.zig:135-148): primitives -> conserved. */
+/* p2u (koral/p2u.zig:135-148): primitives -> conserved.  The primitives'
+ * velocities are VELR, whose conversion to u^mu cannot fail
+ * (koral/relele.zig:360-367 convertBothVelrG), so p2u has no per-cell error
+ * to report and the batch has no error output. */
 int kz_p2u_batch(int64_t n, const double *pp, const kz_geom *g, double gam, double *uu);
 
 /* u2pMhd (koral/solve/invert.zig:422-472): pp in = guess, out = result. */

@@ -52,7 +52,7 @@ int kz_host_puffy_prepinit(const kz_grid *g, double a, double mksr0, double mksh
                            double atm_tgas, double atm_trad_init, double lt_kappa, const double *gc,
                            const double *gcon, double *p);
 
-/* calc_BfromA with overwrite, 2D (koral/sim/ct.zig:184-334) */
+/* calc_BfromA with overwrite, 2D and 3D (koral/sim/ct.zig:184-334) */
 int kz_host_calc_b_from_a(const kz_grid *g, const double *gc, double *p);
 
 /* seeded multiplicative perturbation of the domain state (see kz_host.cpp) */
