@@ -45,6 +45,29 @@ int kz_implicit_batch_solve(kz_implicit_batch *b, double *ms);
  * outcome per cell, counts[3] = failures, Newton iterations, solves. */
 int kz_implicit_batch_get(kz_implicit_batch *b, double *uu, double *pp, kz_implicit_result *r, uint64_t *counts);
 
+/* No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  kz_implicit_batch_solve over the
+ * caller's device arrays, with every working buffer (geometry planes, SoA
+ * states, the Newton reference states, the compacted lists, the counters)
+ * carved out of one static device array of this library, about 24 MB per
+ * GPU, held until the process ends.  Calls no hipMalloc, hipFree or other
+ * allocating routine; enqueues on `stream` (a hipStream_t; NULL: the null
+ * stream) and returns without waiting.
+ *
+ * All pointers are device memory: pp_in 13 doubles per cell and g one
+ * kz_geom per cell (only g's upper triangle is read: the metric must be
+ * symmetric, as kz_implicit_batch_set requires), dt per cell; outputs uu and
+ * pp 13 doubles per cell (nullable; a failed cell gets its input pp and a
+ * zero uu), r per cell, counts[3] = failures, Newton iterations, solves
+ * (nullable).  Results equal kz_implicit_batch_solve's bit for bit.
+ * n above KZ_NOALLOC_MAX_CELLS is rejected with KZ_E_INVALID_CONFIG.  Calls
+ * that overlap on one GPU share its scratch and race: serialise them. */
+#define KZ_NOALLOC_MAX_CELLS 32768
+int kz_implicit_solve_noalloc(int64_t n, const double *pp_in, const kz_geom *g, const double *dt, double *uu,
+                              double *pp, kz_implicit_result *r, uint64_t *counts, const kz_physics *phys,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/kz_step.h"
@@ -3611,3 +3612,119 @@ int kz_implicit_batch_get(kz_implicit_batch *b, double *uu, double *pp, kz_impli
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// The implicit batch solve without device allocation (kz_implicit_solve_noalloc):
+// the same kernels as kz_implicit_batch_solve over the caller's device
+// arrays, with every working buffer carved out of one static device array.
+// Per cell: the geometry planes (GEO_NC + 1), the SoA uu and pp (2 x 13),
+// the Newton reference state and start (2 x 13), the reference-state
+// scalars (IMP_NAUX) as doubles, and five int32 list entries.
+#define NA_DBL_PER_CELL (GEO_NC + 1 + 4 * NV + IMP_NAUX)
+#define NA_INT_PER_CELL 5
+#define NA_NCNT 12  // cnt[4] nslow ncnt[2] rc[2] qpos[3]
+static constexpr size_t NA_BYTES =
+    ((size_t)KZ_NOALLOC_MAX_CELLS * (NA_DBL_PER_CELL * 8 + NA_INT_PER_CELL * 4) + NA_NCNT * 8 + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// AoS caller inputs -> the SoA planes the solve reads; outputs cleared
+__global__ void k_na_pack(long n, const double *pp_aos, const kz_geom *g, double *gp, double *pp, double *uu,
+                          kz_implicit_result *res, unsigned long long *ctr) {
+  long t = TID;
+  if (t < NA_NCNT) ctr[t] = 0;
+  if (t >= n) return;
+  const kz_geom &k = g[t];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = a; c < 4; ++c) gp[tri(a, c) * n + t] = k.gg[a][c];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gp[(10 + a * 4 + c) * n + t] = k.GG[a][c];
+  gp[26 * n + t] = k.gdet;
+  gp[27 * n + t] = k.gttpert;
+  gp[28 * n + t] = k.alpha;
+  gp[GEO_NC * n + t] = k.xx[1];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    pp[iv * n + t] = pp_aos[t * NV + iv];
+    uu[iv * n + t] = 0.0;
+  }
+  res[t] = kz_implicit_result{0, -1, 0, WP_RAD, WE_ENERGY, WF_LAB};
+}
+__global__ void k_na_unpack(long n, const double *uu, const double *pp, double *uu_aos, double *pp_aos,
+                            const unsigned long long *cnt, uint64_t *counts) {
+  long t = TID;
+  if (counts && t < 3) counts[t] = cnt[t];
+  if (t >= n * NV) return;
+  long c = t / NV;
+  int iv = t % NV;
+  if (uu_aos) uu_aos[t] = uu[iv * n + c];
+  if (pp_aos) pp_aos[t] = pp[iv * n + c];
+}
+
+namespace {
+struct NaDevice {
+  unsigned char *base = nullptr;  // this device's copy of `scratch`
+  unsigned blocks = 0;            // persistent launch width: one wave per SIMD
+};
+std::mutex na_mu;
+NaDevice na_dev[64];
+}  // namespace
+
+extern "C" int kz_implicit_solve_noalloc(int64_t n, const double *pp_in, const kz_geom *g, const double *dt,
+                                         double *uu, double *pp, kz_implicit_result *res, uint64_t *counts,
+                                         const kz_physics *phys, void *stream) {
+  if (n < 0 || n > KZ_NOALLOC_MAX_CELLS) {
+    fprintf(stderr, "kz_implicit_solve_noalloc: %lld cells, at most %d fit the static scratch\n", (long long)n,
+            KZ_NOALLOC_MAX_CELLS);
+    return KZ_E_INVALID_CONFIG;
+  }
+  if (!phys || !phys->has_opac || phys->opac.synchrotron_bridge) return phys ? KZ_E_UNSUPPORTED : KZ_E_INVALID_CONFIG;
+  if (n == 0) return KZ_OK;
+  if (!pp_in || !g || !dt || !res) return KZ_E_INVALID_CONFIG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return KZ_E_HIP;
+  NaDevice nd;
+  {
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_dev[dev].base) {
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return KZ_E_HIP;
+      int ncu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return KZ_E_HIP;
+      na_dev[dev].blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);
+      na_dev[dev].base = (unsigned char *)ptr;
+    }
+    nd = na_dev[dev];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  // carve the scratch: doubles first, then the int32 lists, then counters
+  double *dp = (double *)nd.base;
+  double *gp = dp, *suu = gp + (size_t)(GEO_NC + 1) * n, *spp = suu + (size_t)NV * n;
+  ImpWork w{};
+  w.uu00 = spp + (size_t)NV * n;
+  w.pp0 = w.uu00 + (size_t)NV * n;
+  w.aux = w.pp0 + (size_t)NV * n;
+  int32_t *ip = (int32_t *)(w.aux + (size_t)IMP_NAUX * n);
+  w.slow = ip;
+  w.list = ip + n;
+  w.rl = ip + 3 * n;
+  unsigned long long *ctr = (unsigned long long *)(((uintptr_t)(ip + 5 * n) + 7) & ~(uintptr_t)7);
+  unsigned long long *cnt = ctr;
+  w.nslow = ctr + 4;
+  w.ncnt = ctr + 5;
+  w.rc = ctr + 7;
+  w.qpos = ctr + 9;
+  kz_physics ph = *phys;
+  k_na_pack<<<nblk(n < NA_NCNT ? NA_NCNT : n, 256), 256, 0, st>>>(n, pp_in, g, gp, spp, suu, res, ctr);
+  const BatchCells cells{n, gp, dt, suu, spp, res, cnt};
+  const unsigned nb = nblk(n, 64);
+  k_imp_prep<BatchCells><<<nb, 64, 0, st>>>(cells, w, ph);
+  const unsigned np = nd.blocks < nb + 1 ? nd.blocks : nb + 1;
+  k_imp_rung0_persist<BatchCells><<<np, 64, 0, st>>>(cells, w, ph);
+  k_imp_ladder_persist<BatchCells><<<np, 64, 0, st>>>(cells, w, ph);
+  k_imp_slow<BatchCells><<<nb, 64, 0, st>>>(cells, w, ph, 1);
+  k_na_unpack<<<nblk(n * NV, 256), 256, 0, st>>>(n, suu, spp, uu, pp, cnt, counts);
+  return hipGetLastError() == hipSuccess ? KZ_OK : KZ_E_HIP;
+}

@@ -75,6 +75,8 @@ _SIGS = {
     "kz_implicit_batch_set": (C.c_int, [C.c_void_p, D, GEOM, D]),
     "kz_implicit_batch_solve": (C.c_int, [C.c_void_p, D]),
     "kz_implicit_batch_get": (C.c_int, [C.c_void_p, D, D, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "kz_implicit_solve_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.POINTER(T.kz_physics), C.c_void_p]),
     # kz_cellops.h
     "kz_p2u_batch": (C.c_int, [C.c_int64, D, GEOM, C.c_double, D]),
     "kz_u2p_mhd_batch": (C.c_int, [C.c_int64, D, D, GEOM, C.c_double, C.POINTER(T.kz_floor_params), C.c_void_p]),
