@@ -163,7 +163,7 @@ __device__ __forceinline__ bool polar_row(const kz_physics &ph, const GridD &g, 
 // flux and the first Newton rung lose more to spills than they gain.
 #define KZ_WPE(n) __attribute__((amdgpu_waves_per_eu(n)))
 #ifndef KZ_WPE_FACE
-#define KZ_WPE_FACE 1
+#define KZ_WPE_FACE 2
 #endif
 #ifndef KZ_WPE_UPDATE
 #define KZ_WPE_UPDATE 2
@@ -780,17 +780,32 @@ __device__ __forceinline__ void add_visc_flux(const double rv[4], const double *
 #pragma unroll
   for (int nu = 0; nu < 4; ++nu) ff[EE + nu] += gdetu * dampfac * rv[nu];
 }
+// Face fluxes in two phases per block of 64 faces.  Phase 1: one thread per
+// face side reconstructs that side's state (the left state of face i is cell
+// i-1's right PPM value, the right state cell i's left one), applies the face
+// floors and forms its flux fFluxPrime and conserved vector p2u (with the
+// viscous flux when enabled), and leaves both in LDS.  Phase 2: the block's
+// threads combine the two sides of each face into the LAXF flux, one slot per
+// thread.  A side is exactly the computation the reference does for that
+// face state, so the fluxes and the floor guard count are the reference's;
+// holding one state per thread instead of two halves the live registers.
+struct FaceShared {
+  double f[2][NV][64];  // [side][slot][face] flux of each side
+  double u[2][NV][64];  // conserved vector of each side
+  double ag[64], agr[64];
+  long fi[64];
+  int dim[64];
+};
 template <int DIM, bool VISC>
-__device__ __forceinline__ void face_flux(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
+__device__ __forceinline__ void face_side(const GridD &g, const Dev &d, const kz_physics &ph, long t, int side,
+                                          int slot, FaceShared &sh) {
   constexpr int dim = DIM;
   constexpr int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
-  long nd = dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz);
-  long n0 = c0d == 1 ? g.ny : g.nx, n1 = c1d == 1 ? g.ny : g.nz;
+  long n0 = c0d == 1 ? g.ny : g.nx;
   long l0 = n0 > 1 ? -1 : 0, h0 = n0 > 1 ? n0 + 1 : n0;
-  long l1 = n1 > 1 ? -1 : 0, h1 = n1 > 1 ? n1 + 1 : n1;
-  long w0 = h0 - l0, w1 = h1 - l1, wf = nd + 1;
-  if (nd <= 1 || t >= wf * w0 * w1) return;  // a collapsed dimension has no faces
-  // x-innermost thread order (coalesced plane loads)
+  long l1 = (c1d == 1 ? g.ny : g.nz) > 1 ? -1 : 0;
+  long w0 = h0 - l0, wf = (dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz)) + 1;
+  // x-innermost face order (coalesced plane loads)
   long i, c0, c1;
   if (dim == 0) {
     i = t % wf;
@@ -805,68 +820,84 @@ __device__ __forceinline__ void face_flux(const GridD &g, const Dev &d, const kz
   cell[dim] = i;
   cell[c0d] = c0;
   cell[c1d] = c1;
-  long n = g.ncell;
-  long stride = dim == 0 ? 1 : (dim == 1 ? g.sx : (long)g.sx * g.sy);
-  long cf = cidx(g, cell[0], cell[1], cell[2]);
-  double dxs[6];
+  const long n = g.ncell;
+  const long stride = dim == 0 ? 1 : (dim == 1 ? g.sx : (long)g.sx * g.sy);
+  const long cf = cidx(g, cell[0], cell[1], cell[2]);
+  const long cm = cf - stride;
+  // side 0: cell i-1 over stencil i-3..i+1; side 1: cell i over i-2..i+2
+  const long o = side - 3;
+  double dxs[5];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) dxs[k] = csize(g, i - 3 + k, dim);
-  double pl[NV], pr[NV];  // pl: left state at face i (cell i-1's right); pr: right state (cell i's left)
+  for (int k = 0; k < 5; ++k) dxs[k] = csize(g, i + o + k, dim);
+  double ps[NV];
 #pragma unroll
   for (int iv = 0; iv < NV; ++iv) {
     const double *f = d.p + iv * n;
-    double s[6];
+    double s5[5];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = f[cf + (k - 3) * stride];
-    double l0_, r0_, l1_, r1_;
-    ppm(s, dxs, &l0_, &r0_);          // cell i-1: stencil i-3..i+1
-    ppm(s + 1, dxs + 1, &l1_, &r1_);  // cell i:   stencil i-2..i+2
-    pl[iv] = r0_;
-    pr[iv] = l1_;
+    for (int k = 0; k < 5; ++k) s5[k] = f[cf + (o + k) * stride];
+    double lo, ro;
+    ppm(s5, dxs, &lo, &ro);
+    ps[iv] = side == 0 ? ro : lo;
   }
   Geo gm;
   geo_f(g, d, dim, cell[0], cell[1], cell[2], gm);
-  // each face state: floors, flux, conserved (left state first, then right)
   int guard = 0;
-  double fll[NV], flr[NV], ul[NV], ur[NV];
-  check_floors_mhd(pl, gm, ph.gam, ph.floors, &guard);
-  int e1 = flux_prime_d<DIM>(pl, gm, ph.gam, fll);
-  p2u(pl, ul, gm, ph.gam);
-  check_floors_mhd(pr, gm, ph.gam, ph.floors, &guard);
-  int e2 = flux_prime_d<DIM>(pr, gm, ph.gam, flr);
-  p2u(pr, ur, gm, ph.gam);
-  if (e1 | e2) set_err(d.err, E_NANFLUX);
-  long cm = cf - stride;
+  double fl[NV], ul[NV];
+  check_floors_mhd(ps, gm, ph.gam, ph.floors, &guard);
+  if (flux_prime_d<DIM>(ps, gm, ph.gam, fl)) set_err(d.err, E_NANFLUX);
+  p2u(ps, ul, gm, ph.gam);
   if (VISC) {
     double rv[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       rv[j] = 0.5 * (d.rijvisc[((DIM + 1) * 4 + j) * n + cf] + d.rijvisc[((DIM + 1) * 4 + j) * n + cm]);
-    add_visc_flux<DIM>(rv, ul, gm, g, ph, fll);
-    add_visc_flux<DIM>(rv, ur, gm, g, ph, flr);
+    add_visc_flux<DIM>(rv, ul, gm, g, ph, fl);
   }
-  double ag = kzm_fmax(d.scal[(SC_AHD + dim) * n + cf], d.scal[(SC_AHD + dim) * n + cm]);
-  double agr = kzm_fmax(d.scal[(SC_ARAD + dim) * n + cf], d.scal[(SC_ARAD + dim) * n + cm]);
-  long fi = fidx(g, dim, cell[0], cell[1], cell[2]);
-  long nfd = g.nface[dim];
-  double *fo = d.flb[dim];
 #pragma unroll
-  for (int iv = 0; iv < NV; ++iv) fo[iv * nfd + fi] = laxf(fll[iv], flr[iv], ul[iv], ur[iv], iv >= EE ? agr : ag);
+  for (int iv = 0; iv < NV; ++iv) {
+    sh.f[side][iv][slot] = fl[iv];
+    sh.u[side][iv][slot] = ul[iv];
+  }
+  if (side == 0) {
+    sh.ag[slot] = kzm_fmax(d.scal[(SC_AHD + dim) * n + cf], d.scal[(SC_AHD + dim) * n + cm]);
+    sh.agr[slot] = kzm_fmax(d.scal[(SC_ARAD + dim) * n + cf], d.scal[(SC_ARAD + dim) * n + cm]);
+    sh.fi[slot] = fidx(g, dim, cell[0], cell[1], cell[2]);
+    sh.dim[slot] = dim;
+  }
   if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
 }
 // One launch for the faces of every active dimension (faces of different
-// dimensions are independent): threads [0, nf0) do x faces, then y, then z.
-// VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
+// dimensions are independent): faces [0, nf0) are x faces, then y, then z.
+// Block b owns faces [64 b, 64 b + 64); thread j is side (j & 1) of its face
+// 64 b + (j >> 1).  VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
 template <bool VISC>
 __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, Dev d, kz_physics ph) {
-  long t = TID;
-  long n0 = face_count(g, 0), n1 = face_count(g, 1), n2 = face_count(g, 2);
-  if (t >= n0 + n1 + n2) return;  // the launch's tail threads
-  int dim = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
-  t -= dim == 0 ? 0 : (dim == 1 ? n0 : n0 + n1);
-  if (dim == 0) face_flux<0, VISC>(g, d, ph, t);
-  else if (dim == 1) face_flux<1, VISC>(g, d, ph, t);
-  else face_flux<2, VISC>(g, d, ph, t);
+  __shared__ FaceShared sh;
+  const int j = threadIdx.x;
+  const int slot = j >> 1, side = j & 1;
+  const long n0 = face_count(g, 0), n1 = face_count(g, 1), n2 = face_count(g, 2);
+  long t = (long)blockIdx.x * 64 + slot;
+  if (side == 0) sh.dim[slot] = -1;  // no face (the launch's tail)
+  if (t < n0 + n1 + n2) {
+    int dim = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
+    t -= dim == 0 ? 0 : (dim == 1 ? n0 : n0 + n1);
+    if (dim == 0) face_side<0, VISC>(g, d, ph, t, side, slot, sh);
+    else if (dim == 1) face_side<1, VISC>(g, d, ph, t, side, slot, sh);
+    else face_side<2, VISC>(g, d, ph, t, side, slot, sh);
+  }
+  __syncthreads();
+  // LAXF: thread j combines slots [0, 7) (j < 64) or [7, 13) of face j & 63
+  const int fs = j & 63;
+  const int dim = sh.dim[fs];
+  if (dim < 0) return;
+  const long fi = sh.fi[fs];
+  const long nfd = g.nface[dim];
+  double *fo = d.flb[dim];
+  const double ag = sh.ag[fs], agr = sh.agr[fs];
+  const int iv0 = j < 64 ? 0 : 7, iv1 = j < 64 ? 7 : NV;
+  for (int iv = iv0; iv < iv1; ++iv)
+    fo[iv * nfd + fi] = laxf(sh.f[0][iv][fs], sh.f[1][iv][fs], sh.u[0][iv][fs], sh.u[1][iv][fs], iv >= EE ? agr : ag);
 }
 
 
@@ -2545,8 +2576,8 @@ static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
       nf += (nd3[dim] + 1) * (n0 > 1 ? n0 + 2 : n0) * (n1 > 1 ? n1 + 2 : n1);
     }
     TimedLaunch tl(s, KZ_K_FACE_FLUX);
-    if (s->ph.radviscosity) k_face_flux<true><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
-    else k_face_flux<false><<<nblk(nf, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
+    if (s->ph.radviscosity) k_face_flux<true><<<nblk(nf, 64), 128, 0, s->stream>>>(g, s->d, s->ph);
+    else k_face_flux<false><<<nblk(nf, 64), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
   long nc = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   {
