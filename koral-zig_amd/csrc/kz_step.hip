@@ -183,6 +183,12 @@ __device__ __forceinline__ bool polar_row(const kz_physics &ph, const GridD &g, 
 #ifndef KZ_WPE_BC
 #define KZ_WPE_BC 1
 #endif
+// Width of the persistent implicit launches in blocks of one wave per CU
+// (4: one wave per SIMD, which is what the first-rung registers allow to be
+// resident at once); chosen by the tuner on the independent-cell batch.
+#ifndef KZ_PERSIST_PER_CU
+#define KZ_PERSIST_PER_CU 4
+#endif
 
 // ---------------------------------------------------------------------------
 // p2u over domain cells (Sim.initCell / postinit refresh)
@@ -997,113 +1003,132 @@ __global__ void k_emf(GridD g, Dev d) {
 __device__ __forceinline__ double emf_at(const GridD &g, const Dev &d, int c, long ix, long iy, long iz) {
   return d.emf[c][eidx(g, ix, iy, iz)];
 }
+// Two threads per cell: side 0 forms the metric source terms, side 1 the
+// flux differences; both leave their 13 rows in LDS and the update
+// u + du + ms dt is then written half by each (the same expression as one
+// thread evaluating both).
 __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, Dev d, kz_physics ph, int sslot) {
-  long t = TID;
-  if (t >= (long)g.nx * g.ny * g.nz) return;
-  long ix = t % g.nx, iy = (t / g.nx) % g.ny, iz = t / ((long)g.nx * g.ny);
-  long n = g.ncell;
-  long c = cidx(g, ix, iy, iz);
-  double dt = d.ss[sslot];
-  Geo gm;
-  geo_c(g, d, ix, iy, iz, gm);
-  double pp[NV];
-  ld(d.p, n, c, pp);
-  double ucon[4], ucov[4], bcon[4], bcov[4], bsq, urf[4];
-  ucon_ucov(pp, gm, ucon, ucov);
-  bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
-  {
-    double v[4] = {0, pp[FX], pp[FY], pp[FZ]};
-    velr_to_vel4(v, gm, urf);
+  __shared__ double sms[NV][64], sdu[NV][64];
+  const int slot = threadIdx.x >> 1, side = threadIdx.x & 1;
+  const long t = (long)blockIdx.x * 64 + slot;
+  const bool live = t < (long)g.nx * g.ny * g.nz;
+  long ix = 0, iy = 0, iz = 0, c = 0;
+  const long n = g.ncell;
+  const double dt = d.ss[sslot];
+  if (live) {
+    ix = t % g.nx; iy = (t / g.nx) % g.ny; iz = t / ((long)g.nx * g.ny);
+    c = cidx(g, ix, iy, iz);
   }
-  const double rho = pp[RHO], u = pp[UU], erf = pp[EE];
-  const double pr = (ph.gam - 1.0) * u;
-  const double eta = (rho + u + pr) + bsq;
-  const double ptot = pr + 0.5 * bsq;
-  double ms[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) ms[i] = 0;
-  const double *kr = d.kris;
-  const double gdetu = gm.gdet;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    double trow[4], rrow[4], tk[4], rk[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      trow[m] = eta * ucon[k] * ucon[m] + ptot * gm.G[k][m] - bcon[k] * bcon[m];  // hydro.zig:87-112
-      rrow[m] = four_third * erf * urf[k] * urf[m] + one_third * erf * gm.G[k][m];  // radiation.zig:79-98
+  if (live && side == 0) {
+    Geo gm;
+    geo_c(g, d, ix, iy, iz, gm);
+    double pp[NV];
+    ld(d.p, n, c, pp);
+    double ucon[4], ucov[4], bcon[4], bcov[4], bsq, urf[4];
+    ucon_ucov(pp, gm, ucon, ucov);
+    bcon_bcov_bsq(pp[B1], pp[B2], pp[B3], ucon, ucov, gm, bcon, bcov, &bsq);
+    {
+      double v[4] = {0, pp[FX], pp[FY], pp[FZ]};
+      velr_to_vel4(v, gm, urf);
     }
+    const double rho = pp[RHO], u = pp[UU], erf = pp[EE];
+    const double pr = (ph.gam - 1.0) * u;
+    const double eta = (rho + u + pr) + bsq;
+    const double ptot = pr + 0.5 * bsq;
+    double ms[NV];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      double st = 0, sr = 0;
+    for (int i = 0; i < NV; ++i) ms[i] = 0;
+    const double *kr = d.kris;
+    const double gdetu = gm.gdet;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double trow[4], rrow[4], tk[4], rk[4];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        st += trow[m] * gm.g[m][l];
-        sr += rrow[m] * gm.g[m][l];
+        trow[m] = eta * ucon[k] * ucon[m] + ptot * gm.G[k][m] - bcon[k] * bcon[m];  // hydro.zig:87-112
+        rrow[m] = four_third * erf * urf[k] * urf[m] + one_third * erf * gm.G[k][m];  // radiation.zig:79-98
       }
-      tk[l] = st;
-      rk[l] = sr;
-    }
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+      for (int l = 0; l < 4; ++l) {
+        double st_ = 0, sr = 0;
 #pragma unroll
-      for (int nu = 0; nu < 4; ++nu) {
-        const int a = nu < k ? nu : k, b = nu < k ? k : nu;
-        double K = kr[(l * 10 + tri(a, b)) * n + c];
-        ms[UU + nu] += gdetu * tk[l] * K;
-        ms[EE + nu] += gdetu * rk[l] * K;
+        for (int m = 0; m < 4; ++m) {
+          st_ += trow[m] * gm.g[m][l];
+          sr += rrow[m] * gm.g[m][l];
+        }
+        tk[l] = st_;
+        rk[l] = sr;
       }
-  }
-  double dx = csize(g, ix, 0), dy = csize(g, iy, 1), dz = csize(g, iz, 2);
-  long fx0 = fidx(g, 0, ix, iy, iz), fx1 = fidx(g, 0, ix + 1, iy, iz);
-  long fy0 = fidx(g, 1, ix, iy, iz), fy1 = fidx(g, 1, ix, iy + 1, iz);
-  long fz0 = fidx(g, 2, ix, iy, iz), fz1 = fidx(g, 2, ix, iy, iz + 1);
-  long m0 = g.nface[0], m1 = g.nface[1], m2 = g.nface[2];
-  const bool xa = g.nx > 1, ya = g.ny > 1, za = g.nz > 1;
-  // CT-rebuilt B fluxes (ct.zig:160-175) at the six faces of the cell
-  double bx0[3], bx1[3], by0[3], by1[3], bz0[3], bz1[3];
-  for (int q = 0; q < 2; ++q) {
-    long jx = ix + q, jy = iy + q, jz = iz + q;
-    double *fxb = q ? bx1 : bx0, *fyb = q ? by1 : by0, *fzb = q ? bz1 : bz0;
-    if (xa) {
-      fxb[0] = 0;
-      fxb[1] = 0.5 * (emf_at(g, d, 2, jx, iy, iz) + emf_at(g, d, 2, jx, iy + 1, iz));
-      fxb[2] = -0.5 * (emf_at(g, d, 1, jx, iy, iz) + emf_at(g, d, 1, jx, iy, iz + 1));
-    } else {
-      long f = q ? fx1 : fx0;
-      for (int k = 0; k < 3; ++k) fxb[k] = d.flb[0][(B1 + k) * m0 + f];
-    }
-    if (ya) {
-      fyb[0] = -0.5 * (emf_at(g, d, 2, ix, jy, iz) + emf_at(g, d, 2, ix + 1, jy, iz));
-      fyb[1] = 0;
-      fyb[2] = 0.5 * (emf_at(g, d, 0, ix, jy, iz) + emf_at(g, d, 0, ix, jy, iz + 1));
-    } else {
-      long f = q ? fy1 : fy0;
-      for (int k = 0; k < 3; ++k) fyb[k] = d.flb[1][(B1 + k) * m1 + f];
-    }
-    if (za) {
-      fzb[0] = 0.5 * (emf_at(g, d, 1, ix, iy, jz) + emf_at(g, d, 1, ix + 1, iy, jz));
-      fzb[1] = -0.5 * (emf_at(g, d, 0, ix, iy, jz) + emf_at(g, d, 0, ix, iy + 1, jz));
-      fzb[2] = 0;
-    } else {
-      long f = q ? fz1 : fz0;
-      for (int k = 0; k < 3; ++k) fzb[k] = d.flb[2][(B1 + k) * m2 + f];
-    }
-  }
 #pragma unroll
-  for (int iv = 0; iv < NV; ++iv) {
-    double fxl, fxr, fyl, fyr, fzl, fzr;
-    if (iv >= B1 && iv <= B3) {
-      fxl = bx0[iv - B1]; fxr = bx1[iv - B1];
-      fyl = by0[iv - B1]; fyr = by1[iv - B1];
-      fzl = bz0[iv - B1]; fzr = bz1[iv - B1];
-    } else {
-      fxl = d.flb[0][iv * m0 + fx0]; fxr = d.flb[0][iv * m0 + fx1];
-      fyl = d.flb[1][iv * m1 + fy0]; fyr = d.flb[1][iv * m1 + fy1];
-      fzl = d.flb[2][iv * m2 + fz0]; fzr = d.flb[2][iv * m2 + fz1];
+      for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) {
+          const int a = nu < k ? nu : k, b = nu < k ? k : nu;
+          double K = kr[(l * 10 + tri(a, b)) * n + c];
+          ms[UU + nu] += gdetu * tk[l] * K;
+          ms[EE + nu] += gdetu * rk[l] * K;
+        }
     }
-    double du = -(fxr - fxl) * dt / dx - (fyr - fyl) * dt / dy - (fzr - fzl) * dt / dz;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sms[i][slot] = ms[i];
+  } else if (live) {
+    double dx = csize(g, ix, 0), dy = csize(g, iy, 1), dz = csize(g, iz, 2);
+    long fx0 = fidx(g, 0, ix, iy, iz), fx1 = fidx(g, 0, ix + 1, iy, iz);
+    long fy0 = fidx(g, 1, ix, iy, iz), fy1 = fidx(g, 1, ix, iy + 1, iz);
+    long fz0 = fidx(g, 2, ix, iy, iz), fz1 = fidx(g, 2, ix, iy, iz + 1);
+    long m0 = g.nface[0], m1 = g.nface[1], m2 = g.nface[2];
+    const bool xa = g.nx > 1, ya = g.ny > 1, za = g.nz > 1;
+    // CT-rebuilt B fluxes (ct.zig:160-175) at the six faces of the cell
+    double bx0[3], bx1[3], by0[3], by1[3], bz0[3], bz1[3];
+    for (int q = 0; q < 2; ++q) {
+      long jx = ix + q, jy = iy + q, jz = iz + q;
+      double *fxb = q ? bx1 : bx0, *fyb = q ? by1 : by0, *fzb = q ? bz1 : bz0;
+      if (xa) {
+        fxb[0] = 0;
+        fxb[1] = 0.5 * (emf_at(g, d, 2, jx, iy, iz) + emf_at(g, d, 2, jx, iy + 1, iz));
+        fxb[2] = -0.5 * (emf_at(g, d, 1, jx, iy, iz) + emf_at(g, d, 1, jx, iy, iz + 1));
+      } else {
+        long f = q ? fx1 : fx0;
+        for (int k = 0; k < 3; ++k) fxb[k] = d.flb[0][(B1 + k) * m0 + f];
+      }
+      if (ya) {
+        fyb[0] = -0.5 * (emf_at(g, d, 2, ix, jy, iz) + emf_at(g, d, 2, ix + 1, jy, iz));
+        fyb[1] = 0;
+        fyb[2] = 0.5 * (emf_at(g, d, 0, ix, jy, iz) + emf_at(g, d, 0, ix, jy, iz + 1));
+      } else {
+        long f = q ? fy1 : fy0;
+        for (int k = 0; k < 3; ++k) fyb[k] = d.flb[1][(B1 + k) * m1 + f];
+      }
+      if (za) {
+        fzb[0] = 0.5 * (emf_at(g, d, 1, ix, iy, jz) + emf_at(g, d, 1, ix + 1, iy, jz));
+        fzb[1] = -0.5 * (emf_at(g, d, 0, ix, iy, jz) + emf_at(g, d, 0, ix, iy + 1, jz));
+        fzb[2] = 0;
+      } else {
+        long f = q ? fz1 : fz0;
+        for (int k = 0; k < 3; ++k) fzb[k] = d.flb[2][(B1 + k) * m2 + f];
+      }
+    }
+#pragma unroll
+    for (int iv = 0; iv < NV; ++iv) {
+      double fxl, fxr, fyl, fyr, fzl, fzr;
+      if (iv >= B1 && iv <= B3) {
+        fxl = bx0[iv - B1]; fxr = bx1[iv - B1];
+        fyl = by0[iv - B1]; fyr = by1[iv - B1];
+        fzl = bz0[iv - B1]; fzr = bz1[iv - B1];
+      } else {
+        fxl = d.flb[0][iv * m0 + fx0]; fxr = d.flb[0][iv * m0 + fx1];
+        fyl = d.flb[1][iv * m1 + fy0]; fyr = d.flb[1][iv * m1 + fy1];
+        fzl = d.flb[2][iv * m2 + fz0]; fzr = d.flb[2][iv * m2 + fz1];
+      }
+      sdu[iv][slot] = -(fxr - fxl) * dt / dx - (fyr - fyl) * dt / dy - (fzr - fzl) * dt / dz;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int iv0 = side ? 7 : 0, iv1 = side ? NV : 7;
+  for (int iv = iv0; iv < iv1; ++iv) {
     double uo = d.u[iv * n + c];
-    d.u[iv * n + c] = uo + du + ms[iv] * dt;
+    d.u[iv * n + c] = uo + sdu[iv][slot] + sms[iv][slot] * dt;
   }
 }
 
@@ -2326,7 +2351,7 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
-      s->persist_blocks = (unsigned)(4 * ncu);
+      s->persist_blocks = (unsigned)(KZ_PERSIST_PER_CU * ncu);
   }
   GridD &G = s->g;
   G.nx = gr->nx; G.ny = gr->ny; G.nz = gr->nz; G.ng = gr->ng;
@@ -2586,7 +2611,7 @@ static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
   }
   {
     TimedLaunch tl(s, KZ_K_UPDATE);
-    k_update<<<nblk(ndom(s), 128), 128, 0, s->stream>>>(g, s->d, s->ph, dt_slot);
+    k_update<<<nblk(ndom(s), 64), 128, 0, s->stream>>>(g, s->d, s->ph, dt_slot);
   }
   launch_calc_u2p(s, with_bc);
 }
@@ -2919,7 +2944,7 @@ int kz_group_create(kz_group **out) {
     kz_group_destroy(G);
     return KZ_E_HIP;
   }
-  G->persist_blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);
+  G->persist_blocks = (unsigned)(ncu > 0 ? KZ_PERSIST_PER_CU * ncu : 1024);
   *out = G;
   return 0;
 }
@@ -3551,7 +3576,7 @@ kz_implicit_batch *kz_implicit_batch_create(int64_t n, const kz_physics *ph, int
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       first_err(e, KZ_E_HIP);
-    b->persist_blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);  // one wave per SIMD
+    b->persist_blocks = (unsigned)(ncu > 0 ? KZ_PERSIST_PER_CU * ncu : 1024);  // one wave per SIMD
   }
   for (auto &x : b->ev) first_err(e, hip_err(hipEventCreate(&x)));
   if (e) {
@@ -3724,7 +3749,7 @@ extern "C" int kz_implicit_solve_noalloc(int64_t n, const double *pp_in, const k
       if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return KZ_E_HIP;
       int ncu = 0;
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return KZ_E_HIP;
-      na_dev[dev].blocks = (unsigned)(ncu > 0 ? 4 * ncu : 1024);
+      na_dev[dev].blocks = (unsigned)(ncu > 0 ? KZ_PERSIST_PER_CU * ncu : 1024);
       na_dev[dev].base = (unsigned char *)ptr;
     }
     nd = na_dev[dev];

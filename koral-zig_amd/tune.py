@@ -4,7 +4,8 @@ Each heavy kernel carries an `amdgpu_waves_per_eu` bound (KZ_WPE_* in
 csrc/kz_step.hip): a larger bound caps the registers per lane so that more
 waves share a SIMD and hide each other's latency, at the price of spills.
 The best bound depends on the kernel, the grid and the compiler, so it is
-measured: this module builds one library per candidate bound (only the
+measured; so is the width of the persistent implicit launches (blocks of
+one wave per CU, KZ_PERSIST_PER_CU), on the independent-cell batch: this module builds one library per candidate bound (only the
 bound of one kernel class differs from the defaults in each), times every
 kernel class of the PUFFY step on the GPU with HIP events in a subprocess
 per library, requires every candidate to leave the state bit for bit where
@@ -34,7 +35,7 @@ ROOT = HERE.parent
 # kernel class (hip.KERNELS name) -> the bound macro that governs it, with the
 # default and the candidates tried
 KNOBS = {
-    "face_flux": ("KZ_WPE_FACE", 1, (1, 2)),
+    "face_flux": ("KZ_WPE_FACE", 2, (1, 2, 3)),
     "update": ("KZ_WPE_UPDATE", 2, (1, 2, 3)),
     "u2p": ("KZ_WPE_U2P", 2, (1, 2, 3)),
     "wavespeeds": ("KZ_WPE_WS", 1, (1, 2, 4)),
@@ -42,12 +43,19 @@ KNOBS = {
     "implicit_rung0": ("KZ_WPE_RUNG0", 1, (1, 2)),
 }
 CLASSES = ("face_flux", "implicit", "u2p", "update", "wavespeeds", "ct", "bc")
+# the persistent implicit launches' width (blocks of one wave per CU): timed
+# on the independent-cell batch, where those launches run
+BATCH_KNOBS = {"KZ_PERSIST_PER_CU": (4, (2, 4, 8))}
 
 
 def variants():
     """(name, {macro: value}) for the default build and each single change."""
     out = [("default", {})]
     for cls, (macro, default, cands) in KNOBS.items():
+        for v in cands:
+            if v != default:
+                out.append((f"{macro}={v}", {macro: v}))
+    for macro, (default, cands) in BATCH_KNOBS.items():
         for v in cands:
             if v != default:
                 out.append((f"{macro}={v}", {macro: v}))
@@ -104,6 +112,32 @@ def measure(lib, nx, ny, steps):
     return {"per": per, "ms_per_step": wall / steps * 1e3, "hash": h}
 
 
+def measure_batch(lib, cells, solves):
+    """ms per solve of the independent-cell implicit batch and a hash of its
+    results, with `lib` loaded in place of the built library."""
+    import importlib.util
+    import numpy as np
+    name = "koral_zig_amd"
+    spec = importlib.util.spec_from_file_location(name, HERE / "__init__.py", submodule_search_locations=[str(HERE)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    from koral_zig_amd import abi_types as T
+    from koral_zig_amd import hip, workloads
+    hip.LIB_PATH = Path(lib)
+    phys = T.physics_puffy()
+    pp, geom, dt = workloads.implicit_batch_inputs(
+        cells, 0x4B5A4D493335355F, phys, lambda p, g: hip.radstate(p, g, phys.gam, phys.opac, 0)[:, 9])
+    b = hip.ImplicitBatch(cells, phys)
+    b.set(pp, geom, dt)
+    b.solve()
+    ms = float(np.median([b.solve() for _ in range(solves)]))
+    uu, pq, res, cnt = b.get()
+    b.close()
+    h = hashlib.sha1(uu.tobytes() + pq.tobytes() + res["iters"].tobytes()).hexdigest()[:16]
+    return {"ms_per_solve": ms, "hash": h, "counts": list(cnt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(16, os.cpu_count() or 1))
@@ -113,10 +147,15 @@ def main():
     ap.add_argument("--out", default=str(HERE / "lib" / "tune"))
     ap.add_argument("--build-only", action="store_true", help="compile the variants into --out and stop")
     ap.add_argument("--prebuilt", action="store_true", help="measure the variants already compiled in --out")
+    ap.add_argument("--cells", type=int, default=1 << 20, help="cells of the batch that times the persistent width")
     ap.add_argument("--measure", help=argparse.SUPPRESS)
+    ap.add_argument("--measure-batch", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.measure:
         print(json.dumps(measure(a.measure, a.nx, a.ny, a.steps)), flush=True)
+        return
+    if a.measure_batch:
+        print(json.dumps(measure_batch(a.measure_batch, a.cells, 3)), flush=True)
         return
     Path(a.out).mkdir(parents=True, exist_ok=True)
     vs = variants()
@@ -146,6 +185,17 @@ def main():
         res[name] = json.loads(r.stdout.strip().splitlines()[-1])
         print(f"{name:18s} {res[name]['ms_per_step']:.4f} ms/step " +
               " ".join(f"{k} {v:.4f}" for k, v in res[name]["per"].items()) + f" {res[name]['hash']}", flush=True)
+    bres = {}
+    for name in ["default"] + [f"{m}={v}" for m, (d, cs) in BATCH_KNOBS.items() for v in cs if v != d]:
+        r = subprocess.run([sys.executable, str(Path(__file__)), "--measure-batch", str(libs[name]), "--cells",
+                            str(a.cells)], capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"measuring the batch with {name} failed:\n{r.stderr[-2000:]}")
+        bres[name] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(f"batch {name:20s} {bres[name]['ms_per_solve']:.3f} ms/solve {bres[name]['hash']}", flush=True)
+    bad = [n for n, r in bres.items() if r["hash"] != bres["default"]["hash"]]
+    if bad:
+        raise RuntimeError(f"variants changed the batch results: {bad}")
     ref = res["default"]["hash"]
     bad = [n for n, r in res.items() if r["hash"] != ref]
     if bad:
@@ -163,6 +213,14 @@ def main():
             best = default
         pick[macro] = best
         print(f"{macro:14s} default {default}: " + ", ".join(f"{v} -> {t:.4f} ms" for v, t in sorted(times.items())) +
+              f"  best {best}")
+    for macro, (default, cands) in BATCH_KNOBS.items():
+        times = {v: bres["default" if v == default else f"{macro}={v}"]["ms_per_solve"] for v in cands}
+        best = min(times, key=times.get)
+        if times[best] > 0.99 * times[default]:
+            best = default
+        pick[macro] = best
+        print(f"{macro:14s} default {default}: " + ", ".join(f"{v} -> {t:.3f} ms" for v, t in sorted(times.items())) +
               f"  best {best}")
     print(json.dumps({"selected": pick}))
 
