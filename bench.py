@@ -199,7 +199,7 @@ def bench_implicit_batch(args):
         "config": {"workload": f"independent-cell implicit batch, {n} cells (rho, T_gas, E_hat/u, |F|/E, tau, "
                                f"MKS2 a=0 r in [2.5, 50] spread as specified in koral_zig_amd.workloads)",
                    "cells": n, "seed": args.seed},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "implicit",
                      "avg_launch_ms": avg_ms, "launches": len(kms)},
         "cpu_baseline": cpu,
