@@ -46,6 +46,9 @@ int kzo_check_floors_rad_batch(int64_t n, double *pp, const kz_geom *g, const kz
 int kzo_solve_implicit_lab_batch(int64_t n, double *uu, double *pp, const kz_geom *g, const double *dt, double gam,
                                  const kz_rad_params *rp, const kz_opac_params *op, const kz_implicit_params *ip,
                                  kz_implicit_result *r) {
+  /* records are independent: the oracle's thread count splits them, each
+   * solved exactly as serially */
+#pragma omp parallel for num_threads(kzo_get_threads()) schedule(dynamic, 256)
   for (int64_t i = 0; i < n; ++i) {
     GEO(i);
     r[i] = kzo_solve_implicit_lab(uu + i * NV, pp + i * NV, &gm, dt[i], gam, rp, op, ip);
