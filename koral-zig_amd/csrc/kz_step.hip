@@ -248,149 +248,150 @@ __device__ void geo_from_rec(const double *rec, Geo &o) {
   o.gttpert = 0;
   o.x1 = 0;
 }
-// puffy.zig:1089-1166 (xhi) / :1167-1171 (xlo); t < nz*ny*ngx*2
+// puffy.zig:1089-1166 (xhi) / :1167-1171 (xlo): the primitives of x ghost
+// (ix, iy, iz) from the domain edge cell of its row; geom is the ghost's
+// centre geometry.  Returns an error code, E_OK on success.
+__device__ int bc_x_prims(const GridD &g, const Dev &d, long ix, long iy, long iz, const Geo &geom, double *pp) {
+  const long nx = g.nx;
+  if (ix < 0) {
+    ld(d.p, g.ncell, cidx(g, 0, iy, iz), pp);
+    return E_OK;
+  }
+  ld(d.p, g.ncell, cidx(g, nx - 1, iy, iz), pp);
+  const double *rec = d.bcx + (((iz * g.ny) + iy) * g.ng + (ix - nx)) * BCX_REC;
+  Geo gbl;
+  geo_from_rec(rec, gbl);
+  double rghost = rec[32], rbound = rec[33];
+  const double *J12 = rec + 34, *J21 = rec + 50;
+  trans_pmhd(pp, geom, gbl, J12);
+  double s1 = rbound * rbound / rghost / rghost, s2 = rbound / rghost;
+  pp[RHO] *= s1; pp[UU] *= s1;
+  pp[B1] *= s1; pp[B2] *= s2; pp[B3] *= s2;
+  pp[VY] *= s1; pp[VZ] *= s1;
+  pp[EE] *= s1; pp[FY] *= s1; pp[FZ] *= s1;
+  trans_pmhd(pp, gbl, geom, J21);
+  double uc[4] = {0.0, pp[VX], pp[VY], pp[VZ]};
+  velr_to_vel4(uc, geom, uc);
+  tvec(J12, uc, uc);
+  if (uc[1] < 0.0) {
+    uc[1] = 0.0;
+    tvec(J21, uc, uc);
+    int e = vel4_to_velr(uc, geom, false, uc);
+    if (e) return e;
+    pp[VX] = uc[1]; pp[VY] = uc[2]; pp[VZ] = uc[3];
+  }
+  double ur[4] = {0.0, pp[FX], pp[FY], pp[FZ]};
+  velr_to_vel4(ur, geom, ur);
+  tvec(J12, ur, ur);
+  if (ur[1] < 0.0) {
+    ur[1] = 0.0;
+    tvec(J21, ur, ur);
+    int e = vel4_to_velr(ur, geom, false, ur);
+    if (e) return e;
+    pp[FX] = ur[1]; pp[FY] = ur[2]; pp[FZ] = ur[3];
+  }
+  return E_OK;
+}
+// puffy.zig:1172-1187 polar reflection: the primitives of y ghost (ix, iy, iz)
+__device__ void bc_y_prims(const GridD &g, const Dev &d, long ix, long iy, long iz, double *pp) {
+  const long ny = g.ny;
+  const long iiy = iy >= ny ? ny - (iy - ny) - 1 : -iy - 1;
+  ld(d.p, g.ncell, cidx(g, ix, iiy, iz), pp);
+  pp[VY] = -pp[VY];
+  pp[B2] = -pp[B2];
+  pp[FY] = -pp[FY];
+}
+__device__ void store_p2u(const GridD &g, const Dev &d, double gam, long ix, long iy, long iz, const Geo &geom,
+                          double *pp) {
+  double uu[NV];
+  p2u(pp, uu, geom, gam);
+  const long c = cidx(g, ix, iy, iz);
+  st(d.p, g.ncell, c, pp);
+  st(d.u, g.ncell, c, uu);
+}
+// x ghost t < nz*ny*ngx*2; a ghost whose transform fails keeps its old state
 __device__ void bc_x_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
   int side = t % 2;
   long i = (t / 2) % g.ngx + 1;
   long iy = (t / (2 * g.ngx)) % g.ny;
   long iz = t / (2L * g.ngx * g.ny);
-  long nx = g.nx;
-  long ix = side ? nx - 1 + i : -i;
+  long ix = side ? g.nx - 1 + i : -i;
   double pp[NV];
   Geo geom;
   geo_c(g, d, ix, iy, iz, geom);
-  if (side == 0) {
-    ld(d.p, g.ncell, cidx(g, 0, iy, iz), pp);
-  } else {
-    ld(d.p, g.ncell, cidx(g, nx - 1, iy, iz), pp);
-    const double *rec = d.bcx + (((iz * g.ny) + iy) * g.ng + (ix - nx)) * BCX_REC;
-    Geo gbl;
-    geo_from_rec(rec, gbl);
-    double rghost = rec[32], rbound = rec[33];
-    const double *J12 = rec + 34, *J21 = rec + 50;
-    trans_pmhd(pp, geom, gbl, J12);
-    double s1 = rbound * rbound / rghost / rghost, s2 = rbound / rghost;
-    pp[RHO] *= s1; pp[UU] *= s1;
-    pp[B1] *= s1; pp[B2] *= s2; pp[B3] *= s2;
-    pp[VY] *= s1; pp[VZ] *= s1;
-    pp[EE] *= s1; pp[FY] *= s1; pp[FZ] *= s1;
-    trans_pmhd(pp, gbl, geom, J21);
-    double uc[4] = {0.0, pp[VX], pp[VY], pp[VZ]};
-    velr_to_vel4(uc, geom, uc);
-    tvec(J12, uc, uc);
-    if (uc[1] < 0.0) {
-      uc[1] = 0.0;
-      tvec(J21, uc, uc);
-      int e = vel4_to_velr(uc, geom, false, uc);
-      if (e) { set_err(d.err, e); return; }
-      pp[VX] = uc[1]; pp[VY] = uc[2]; pp[VZ] = uc[3];
-    }
-    double ur[4] = {0.0, pp[FX], pp[FY], pp[FZ]};
-    velr_to_vel4(ur, geom, ur);
-    tvec(J12, ur, ur);
-    if (ur[1] < 0.0) {
-      ur[1] = 0.0;
-      tvec(J21, ur, ur);
-      int e = vel4_to_velr(ur, geom, false, ur);
-      if (e) { set_err(d.err, e); return; }
-      pp[FX] = ur[1]; pp[FY] = ur[2]; pp[FZ] = ur[3];
-    }
+  int e = bc_x_prims(g, d, ix, iy, iz, geom, pp);
+  if (e) {
+    set_err(d.err, e);
+    return;
   }
-  double uu[NV];
-  p2u(pp, uu, geom, ph.gam);
-  long c = cidx(g, ix, iy, iz);
-  st(d.p, g.ncell, c, pp);
-  st(d.u, g.ncell, c, uu);
+  store_p2u(g, d, ph.gam, ix, iy, iz, geom, pp);
 }
-// puffy.zig:1172-1187 polar reflection; t < nz*nx*ngy*2
+// y ghost t < nz*nx*ngy*2
 __device__ void bc_y_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
   int side = t % 2;
   long i = (t / 2) % g.ngy + 1;
   long ix = (t / (2 * g.ngy)) % g.nx;
   long iz = t / (2L * g.ngy * g.nx);
-  long ny = g.ny;
-  long iy = side ? ny - 1 + i : -i;
-  long iiy = side ? ny - (iy - ny) - 1 : -iy - 1;
-  double pp[NV], uu[NV];
-  ld(d.p, g.ncell, cidx(g, ix, iiy, iz), pp);
-  pp[VY] = -pp[VY];
-  pp[B2] = -pp[B2];
-  pp[FY] = -pp[FY];
+  long iy = side ? g.ny - 1 + i : -i;
+  double pp[NV];
+  bc_y_prims(g, d, ix, iy, iz, pp);
   Geo geom;
   geo_c(g, d, ix, iy, iz, geom);
-  p2u(pp, uu, geom, ph.gam);
-  long c = cidx(g, ix, iy, iz);
-  st(d.p, g.ncell, c, pp);
-  st(d.u, g.ncell, c, uu);
+  store_p2u(g, d, ph.gam, ix, iy, iz, geom, pp);
 }
-// x and y ghost cells in one launch: the x pass reads only domain cells of
-// rows 0..ny-1 and writes x ghosts, the y pass reads only domain cells and
-// writes y ghosts, so the two are independent (bc.zig:40-73 runs them in
-// sequence).
 // bc.zig:286-377 fillCorners2d.  Per corner quadrant the reference copies
-// 2(ng-1) cells from x/y ghosts, averages the diagonal cell next to the
-// domain from an x and a y ghost, then averages the outer diagonal cell from
-// two of the copied cells.  The first group reads no cell of its own group,
-// so it runs one thread per cell; the outer diagonals follow a barrier.
-// Quadrants never read each other's cells.  One block of 64 threads.
-__device__ void cp_p2u(const GridD &g, const Dev &d, double gam, long dx_, long dy_, long sx_, long sy_) {
-  double pp[NV], uu[NV];
-  ld(d.p, g.ncell, cidx(g, sx_, sy_, 0), pp);
-  Geo gm;
-  geo_c(g, d, dx_, dy_, 0, gm);
-  p2u(pp, uu, gm, gam);
-  long c = cidx(g, dx_, dy_, 0);
-  st(d.p, g.ncell, c, pp);
-  st(d.u, g.ncell, c, uu);
+// 2(ng-1) cells from the x and y ghosts next to the corner (the x ghosts of
+// the edge row into the first ghost row, the y ghosts of the edge column
+// into the first ghost column), averages the inner diagonal cell from the x
+// ghost and the y ghost beside it, then the outer diagonal cell from two of
+// the copies.  Every value involved is a ghost value of this setBc, which is
+// a function of domain cells only, so each corner cell is formed by its own
+// thread from those ghost functions, in the same launch as the ghosts: the
+// same primitives, p2u on the corner cell's geometry.  A ghost whose
+// transform fails is not written by this setBc; its old state is used, as
+// the reference's copy reads it.  t < 4 (2 (ng-1) + 2).
+__device__ void corner_x_src(const GridD &g, const Dev &d, long x, long yin, double *pp) {
+  Geo gs;
+  geo_c(g, d, x, yin, 0, gs);
+  if (bc_x_prims(g, d, x, yin, 0, gs, pp)) ld(d.p, g.ncell, cidx(g, x, yin, 0), pp);
 }
-__device__ void avg_p2u(const GridD &g, const Dev &d, double gam, long dx_, long dy_, long ax, long ay, long bx,
-                        long by) {
-  double pa[NV], pb[NV], pp[NV], uu[NV];
-  ld(d.p, g.ncell, cidx(g, ax, ay, 0), pa);
-  ld(d.p, g.ncell, cidx(g, bx, by, 0), pb);
-#pragma unroll
-  for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
-  Geo gm;
-  geo_c(g, d, dx_, dy_, 0, gm);
-  p2u(pp, uu, gm, gam);
-  long c = cidx(g, dx_, dy_, 0);
-  st(d.p, g.ncell, c, pp);
-  st(d.u, g.ncell, c, uu);
-}
-// fillCorners2d's two phases, run by one block of at least 20 threads
-__device__ void bc_corners2d(const GridD &g, const Dev &d, const kz_physics &ph) {
+__device__ void bc_corner2d_cell(const GridD &g, const Dev &d, const kz_physics &ph, long t) {
   const long nx = g.nx, ny = g.ny, ng = g.ng;
-  const double gam = ph.gam;
-  const int per = 2 * (g.ng - 1) + 1;  // copies + inner diagonal per quadrant
-  const int tid = threadIdx.x;
-  if (tid < 4 * per) {
-    const int q = tid / per, k = tid % per;
-    // quadrant q: bit 0 = high y side, bit 1 = high x side
-    const bool hx = q & 2, hy = q & 1;
-    const long xin = hx ? nx - 1 : 0, yin = hy ? ny - 1 : 0;    // edge domain index
-    const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;          // first ghost index
-    if (k < 2 * (g.ng - 1)) {
-      const long i = k / 2;
-      if (k % 2 == 0) {
-        // ghost column beyond x, first ghost row in y: copy from the edge row
-        const long x = hx ? nx + i + 1 : -ng + i;
-        cp_p2u(g, d, gam, x, yg1, x, yin);
-      } else {
-        const long y = hy ? ny + i + 1 : -ng + i;
-        cp_p2u(g, d, gam, xg1, y, xin, y);
-      }
-    } else {
-      avg_p2u(g, d, gam, xg1, yg1, xg1, yin, xin, yg1);
+  const int per = 2 * (int)(ng - 1) + 2;  // copies, inner and outer diagonal
+  const int q = (int)(t / per), k = (int)(t % per);
+  // quadrant q: bit 0 = high y side, bit 1 = high x side
+  const bool hx = q & 2, hy = q & 1;
+  const long xin = hx ? nx - 1 : 0, yin = hy ? ny - 1 : 0;  // edge domain index
+  const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;        // first ghost index
+  const long xg2 = hx ? nx + 1 : -2, yg2 = hy ? ny + 1 : -2;
+  double pp[NV];
+  long dx_, dy_;
+  if (k < 2 * (ng - 1)) {
+    const long i = k / 2;
+    if (k % 2 == 0) {  // (x, yg1) <- x ghost (x, yin)
+      dx_ = hx ? nx + i + 1 : -ng + i;
+      dy_ = yg1;
+      corner_x_src(g, d, dx_, yin, pp);
+    } else {  // (xg1, y) <- y ghost (xin, y)
+      dx_ = xg1;
+      dy_ = hy ? ny + i + 1 : -ng + i;
+      bc_y_prims(g, d, xin, dy_, 0, pp);
     }
+  } else {
+    // inner: x ghost (xg1, yin) and y ghost (xin, yg1); outer: the copies
+    // (xg2, yg1) of x ghost (xg2, yin) and (xg1, yg2) of y ghost (xin, yg2)
+    const bool inner = k == 2 * (ng - 1);
+    dx_ = inner ? xg1 : xg2;
+    dy_ = inner ? yg1 : yg2;
+    double pa[NV], pb[NV];
+    corner_x_src(g, d, dx_, yin, pa);
+    bc_y_prims(g, d, xin, dy_, 0, pb);
+#pragma unroll
+    for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
   }
-  __threadfence_block();
-  __syncthreads();
-  if (tid < 4) {
-    const bool hx = tid & 2, hy = tid & 1;
-    const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;
-    const long xg2 = hx ? nx + 1 : -2, yg2 = hy ? ny + 1 : -2;
-    avg_p2u(g, d, gam, xg2, yg2, xg2, yg1, xg1, yg2);
-  }
+  Geo gm;
+  geo_c(g, d, dx_, dy_, 0, gm);
+  store_p2u(g, d, ph.gam, dx_, dy_, 0, gm, pp);
 }
 
 // z faces, periodic in phi (puffy.zig:237, bc.zig:187-252 .periodic, with its
@@ -537,8 +538,10 @@ __global__ void __launch_bounds__(64) k_bc_edges2(GridD g, Dev d, kz_physics ph)
 
 // setBc (bc.zig:40-73): x ghosts (reading only domain cells of rows
 // 0..ny-1), y ghosts and z ghosts (reading only domain cells) are
-// independent, so one launch does all three; the corner fill, which reads
-// them, follows (k_bc_corners in 2D, k_bc_edges1/2 in 3D).
+// independent, so one launch does all three, and in 2D the corner fill as
+// well (bc_corner2d_cell); in 3D the corner fill, which reads them, follows
+// (k_bc_edges1/2).
+__device__ __forceinline__ long bc_corner_count(const GridD &g) { return g.nz == 1 ? 4L * (2 * (g.ng - 1) + 2) : 0; }
 __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_BC) k_bc(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   long nxg = (long)g.nz * g.ny * g.ngx * 2;
@@ -547,8 +550,8 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_BC) k_bc(GridD g, Dev d, kz_
   if (t < nxg) bc_x_cell(g, d, ph, t);
   else if (t < nxg + nyg) bc_y_cell(g, d, ph, t - nxg);
   else if (t < nxg + nyg + nzg) bc_z_cell(g, d, ph, t - nxg - nyg);
+  else if (t < nxg + nyg + nzg + bc_corner_count(g)) bc_corner2d_cell(g, d, ph, t - nxg - nyg - nzg);
 }
-__global__ void __launch_bounds__(64) k_bc_corners(GridD g, Dev d, kz_physics ph) { bc_corners2d(g, d, ph); }
 
 // ---------------------------------------------------------------------------
 // cellFixup (sim.zig:1027-1133) for one flagged cell: average the fixed slots
@@ -753,16 +756,9 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_WS) k_wavespeeds(GridD g, D
 
 // ---------------------------------------------------------------------------
 // fused face flux: PPM of the two adjacent cells, face floors, fFluxPrime,
-// p2u, LAXF (sim.zig:1284-1503).  One thread per face i in [0, n] of
-// dimension `dim`, cross indices over [-1, n_c + 1) in active dimensions.
-__device__ __forceinline__ long face_count(const GridD &g, int dim) {
-  long nd = dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz);
-  if (nd <= 1) return 0;
-  int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
-  long n0 = c0d == 1 ? g.ny : g.nx, n1 = c1d == 1 ? g.ny : g.nz;
-  long w0 = n0 > 1 ? n0 + 2 : n0, w1 = n1 > 1 ? n1 + 2 : n1;
-  return (nd + 1) * w0 * w1;
-}
+// p2u, LAXF (sim.zig:1284-1503) for faces i in [0, n] of each active
+// dimension, cross indices over [-1, n_c + 1) in the other active ones.
+
 // rijvisc.zig:313-353 faceAvg + radvisc.zig:201-244 addRadViscFlux: row
 // dim+1 of the face-averaged viscous R^i_j (rv), velocity-capped, added to
 // the radiation rows of one face state's flux; uu is that state's p2u.  Of
@@ -786,124 +782,192 @@ __device__ __forceinline__ void add_visc_flux(const double rv[4], const double *
 #pragma unroll
   for (int nu = 0; nu < 4; ++nu) ff[EE + nu] += gdetu * dampfac * rv[nu];
 }
-// Face fluxes in two phases per block of 64 faces.  Phase 1: one thread per
-// face side reconstructs that side's state (the left state of face i is cell
-// i-1's right PPM value, the right state cell i's left one), applies the face
-// floors and forms its flux fFluxPrime and conserved vector p2u (with the
-// viscous flux when enabled), and leaves both in LDS.  Phase 2: the block's
-// threads combine the two sides of each face into the LAXF flux, one slot per
-// thread.  A side is exactly the computation the reference does for that
-// face state, so the fluxes and the floor guard count are the reference's;
-// holding one state per thread instead of two halves the live registers.
+// Face fluxes by tiles.  A block of 128 threads owns up to 64 faces of one
+// dimension whose adjacent cells number at most 64:
+//   x faces: a run of FaceTiles::fx consecutive faces of the x-fastest face
+//   order (a run may continue on the next row);
+//   y and z faces: FT_T consecutive x columns by FT_F consecutive faces
+//   along the dimension.
+// Phase 1: the block reconstructs each of its cells once, both face values
+// of every slot, into LDS (the left state of face i is cell i-1's right
+// value, the right state cell i's left one); the two waves share the slots.
+// Phase 2: one thread per face side takes its state from LDS, applies the
+// face floors and forms its flux fFluxPrime and conserved vector p2u (with
+// the viscous flux when enabled).  Phase 3: the two sides of a face,
+// neighbouring lanes, exchange flux and conserved vector and each writes the
+// LAXF flux of half the slots.  Each side is exactly the reference's
+// computation for that face state, so the fluxes and the floor guard count
+// are the reference's.
+#ifndef KZ_FT_T
+#define KZ_FT_T 8
+#endif
+constexpr int FT_T = KZ_FT_T, FT_F = 64 / FT_T - 1;
+struct FaceTiles {
+  long nb[3];  // blocks of each dimension, launched in the order x, y, z
+  int fx;      // x faces per block
+};
 struct FaceShared {
-  double f[2][NV][64];  // [side][slot][face] flux of each side
-  double u[2][NV][64];  // conserved vector of each side
-  double ag[64], agr[64];
-  long fi[64];
-  int dim[64];
+  double st[2][NV][64];  // [0] left, [1] right face value of each slot of each cell
+};
+// The faces and cells of block b of dimension DIM, numbered locally: face f
+// lies between cells left(f) and right(f).  Coordinates are (ix, iy, iz)
+// with the face index in component DIM.
+template <int DIM>
+struct FaceTile {
+  static constexpr int c0d = DIM == 0 ? 1 : 0, c1d = DIM == 2 ? 1 : 2;
+  long wf, w0, l0, l1;
+  long t0, r0, e0;  // x runs: first face, its row, its left cell's place in rows of wf + 1 cells
+  long x0, i0, c1;  // y/z tiles: first column, first face along DIM, the other cross index
+  int nface, ncell;
+  __device__ FaceTile(const GridD &g, const FaceTiles &ft, long b) {
+    const long n0 = c0d == 1 ? g.ny : g.nx, n1 = c1d == 1 ? g.ny : g.nz;
+    l0 = n0 > 1 ? -1 : 0;
+    w0 = n0 > 1 ? n0 + 2 : n0;
+    l1 = n1 > 1 ? -1 : 0;
+    const long w1 = n1 > 1 ? n1 + 2 : n1;
+    wf = (DIM == 0 ? g.nx : (DIM == 1 ? g.ny : g.nz)) + 1;
+    if (DIM == 0) {
+      t0 = b * ft.fx;
+      const long left = wf * w0 * w1 - t0;
+      nface = (int)(left < ft.fx ? left : ft.fx);
+      r0 = t0 / wf;
+      e0 = r0 * (wf + 1) + t0 % wf;
+      ncell = right(nface - 1) + 1;
+    } else {
+      const long ntx = (w0 + FT_T - 1) / FT_T, nti = (wf + FT_F - 1) / FT_F;
+      x0 = (b % ntx) * FT_T;
+      i0 = ((b / ntx) % nti) * FT_F;
+      c1 = b / (ntx * nti) + l1;
+      nface = FT_T * FT_F;
+      ncell = FT_T * (FT_F + 1);
+    }
+  }
+  __device__ int left(int f) const { return DIM == 0 ? (int)(f + (t0 + f) / wf - r0) : f; }
+  __device__ int right(int f) const { return DIM == 0 ? left(f) + 1 : f + FT_T; }
+  __device__ bool face(int f, long cell[3]) const {
+    if (f >= nface) return false;
+    if (DIM == 0) {
+      const long t = t0 + f;
+      cell[0] = t % wf;
+      cell[1] = (t / wf) % w0 + l0;
+      cell[2] = t / (wf * w0) + l1;
+      return true;
+    }
+    const long x = x0 + f % FT_T, i = i0 + f / FT_T;
+    if (x >= w0 || i >= wf) return false;
+    cell[DIM] = i;
+    cell[c0d] = x + l0;
+    cell[c1d] = c1;
+    return true;
+  }
+  __device__ bool cellk(int k, long cell[3]) const {
+    if (k >= ncell) return false;
+    if (DIM == 0) {
+      const long e = e0 + k, r = e / (wf + 1);
+      cell[0] = e % (wf + 1) - 1;
+      cell[1] = r % w0 + l0;
+      cell[2] = r / w0 + l1;
+      return true;
+    }
+    const long x = x0 + k % FT_T, i = i0 - 1 + k / FT_T;
+    if (x >= w0 || i >= wf) return false;
+    cell[DIM] = i;
+    cell[c0d] = x + l0;
+    cell[c1d] = c1;
+    return true;
+  }
 };
 template <int DIM, bool VISC>
-__device__ __forceinline__ void face_side(const GridD &g, const Dev &d, const kz_physics &ph, long t, int side,
-                                          int slot, FaceShared &sh) {
+__device__ __forceinline__ void face_tile(const GridD &g, const Dev &d, const kz_physics &ph, const FaceTiles &ft,
+                                          long b, FaceShared &sh) {
   constexpr int dim = DIM;
-  constexpr int c0d = dim == 0 ? 1 : 0, c1d = dim == 2 ? 1 : 2;
-  long n0 = c0d == 1 ? g.ny : g.nx;
-  long l0 = n0 > 1 ? -1 : 0, h0 = n0 > 1 ? n0 + 1 : n0;
-  long l1 = (c1d == 1 ? g.ny : g.nz) > 1 ? -1 : 0;
-  long w0 = h0 - l0, wf = (dim == 0 ? g.nx : (dim == 1 ? g.ny : g.nz)) + 1;
-  // x-innermost face order (coalesced plane loads)
-  long i, c0, c1;
-  if (dim == 0) {
-    i = t % wf;
-    c0 = (t / wf) % w0 + l0;
-    c1 = t / (wf * w0) + l1;
-  } else {
-    c0 = t % w0 + l0;
-    i = (t / w0) % wf;
-    c1 = t / (w0 * wf) + l1;
-  }
-  long cell[3];
-  cell[dim] = i;
-  cell[c0d] = c0;
-  cell[c1d] = c1;
+  const FaceTile<DIM> tl(g, ft, b);
+  const int j = threadIdx.x;
   const long n = g.ncell;
   const long stride = dim == 0 ? 1 : (dim == 1 ? g.sx : (long)g.sx * g.sy);
-  const long cf = cidx(g, cell[0], cell[1], cell[2]);
-  const long cm = cf - stride;
-  // side 0: cell i-1 over stencil i-3..i+1; side 1: cell i over i-2..i+2
-  const long o = side - 3;
-  double dxs[5];
+  long cell[3];
+  // phase 1: both face values of cell j & 63 over the stencil along DIM;
+  // the block's first wave takes slots [0, 7), the second [7, 13)
+  const int kc = j & 63;
+  if (tl.cellk(kc, cell)) {
+    const long i = cell[dim];
+    const long c = cidx(g, cell[0], cell[1], cell[2]);
+    double dxs[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) dxs[k] = csize(g, i + o + k, dim);
-  double ps[NV];
+    for (int k = 0; k < 5; ++k) dxs[k] = csize(g, i - 2 + k, dim);
+    const int iv0 = j < 64 ? 0 : 7, iv1 = j < 64 ? 7 : NV;
+    for (int iv = iv0; iv < iv1; ++iv) {
+      const double *f = d.p + iv * n;
+      double s5[5];
 #pragma unroll
-  for (int iv = 0; iv < NV; ++iv) {
-    const double *f = d.p + iv * n;
-    double s5[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) s5[k] = f[cf + (o + k) * stride];
-    double lo, ro;
-    ppm(s5, dxs, &lo, &ro);
-    ps[iv] = side == 0 ? ro : lo;
-  }
-  Geo gm;
-  geo_f(g, d, dim, cell[0], cell[1], cell[2], gm);
-  int guard = 0;
-  double fl[NV], ul[NV];
-  check_floors_mhd(ps, gm, ph.gam, ph.floors, &guard);
-  if (flux_prime_d<DIM>(ps, gm, ph.gam, fl)) set_err(d.err, E_NANFLUX);
-  p2u(ps, ul, gm, ph.gam);
-  if (VISC) {
-    double rv[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      rv[j] = 0.5 * (d.rijvisc[((DIM + 1) * 4 + j) * n + cf] + d.rijvisc[((DIM + 1) * 4 + j) * n + cm]);
-    add_visc_flux<DIM>(rv, ul, gm, g, ph, fl);
-  }
-#pragma unroll
-  for (int iv = 0; iv < NV; ++iv) {
-    sh.f[side][iv][slot] = fl[iv];
-    sh.u[side][iv][slot] = ul[iv];
-  }
-  if (side == 0) {
-    sh.ag[slot] = kzm_fmax(d.scal[(SC_AHD + dim) * n + cf], d.scal[(SC_AHD + dim) * n + cm]);
-    sh.agr[slot] = kzm_fmax(d.scal[(SC_ARAD + dim) * n + cf], d.scal[(SC_ARAD + dim) * n + cm]);
-    sh.fi[slot] = fidx(g, dim, cell[0], cell[1], cell[2]);
-    sh.dim[slot] = dim;
-  }
-  if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
-}
-// One launch for the faces of every active dimension (faces of different
-// dimensions are independent): faces [0, nf0) are x faces, then y, then z.
-// Block b owns faces [64 b, 64 b + 64); thread j is side (j & 1) of its face
-// 64 b + (j >> 1).  VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
-template <bool VISC>
-__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, Dev d, kz_physics ph) {
-  __shared__ FaceShared sh;
-  const int j = threadIdx.x;
-  const int slot = j >> 1, side = j & 1;
-  const long n0 = face_count(g, 0), n1 = face_count(g, 1), n2 = face_count(g, 2);
-  long t = (long)blockIdx.x * 64 + slot;
-  if (side == 0) sh.dim[slot] = -1;  // no face (the launch's tail)
-  if (t < n0 + n1 + n2) {
-    int dim = t < n0 ? 0 : (t < n0 + n1 ? 1 : 2);
-    t -= dim == 0 ? 0 : (dim == 1 ? n0 : n0 + n1);
-    if (dim == 0) face_side<0, VISC>(g, d, ph, t, side, slot, sh);
-    else if (dim == 1) face_side<1, VISC>(g, d, ph, t, side, slot, sh);
-    else face_side<2, VISC>(g, d, ph, t, side, slot, sh);
+      for (int k = 0; k < 5; ++k) s5[k] = f[c + (k - 2) * stride];
+      double lo, ro;
+      ppm(s5, dxs, &lo, &ro);
+      sh.st[0][iv][kc] = lo;
+      sh.st[1][iv][kc] = ro;
+    }
   }
   __syncthreads();
-  // LAXF: thread j combines slots [0, 7) (j < 64) or [7, 13) of face j & 63
-  const int fs = j & 63;
-  const int dim = sh.dim[fs];
-  if (dim < 0) return;
-  const long fi = sh.fi[fs];
+  // phase 2: face side j & 1 of face j >> 1
+  const int slot = j >> 1, side = j & 1;
+  const bool act = tl.face(slot, cell);
+  double fl[NV], ul[NV], ag = 0, agr = 0;
+  long fi = 0;
+  if (act) {
+    const int k = side == 0 ? tl.left(slot) : tl.right(slot);
+    double ps[NV];
+#pragma unroll
+    for (int iv = 0; iv < NV; ++iv) ps[iv] = sh.st[side == 0 ? 1 : 0][iv][k];
+    const long cf = cidx(g, cell[0], cell[1], cell[2]);
+    const long cm = cf - stride;
+    Geo gm;
+    geo_f(g, d, dim, cell[0], cell[1], cell[2], gm);
+    int guard = 0;
+    check_floors_mhd(ps, gm, ph.gam, ph.floors, &guard);
+    if (flux_prime_d<DIM>(ps, gm, ph.gam, fl)) set_err(d.err, E_NANFLUX);
+    p2u(ps, ul, gm, ph.gam);
+    if (VISC) {
+      double rv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        rv[q] = 0.5 * (d.rijvisc[((DIM + 1) * 4 + q) * n + cf] + d.rijvisc[((DIM + 1) * 4 + q) * n + cm]);
+      add_visc_flux<DIM>(rv, ul, gm, g, ph, fl);
+    }
+    ag = kzm_fmax(d.scal[(SC_AHD + dim) * n + cf], d.scal[(SC_AHD + dim) * n + cm]);
+    agr = kzm_fmax(d.scal[(SC_ARAD + dim) * n + cf], d.scal[(SC_ARAD + dim) * n + cm]);
+    fi = fidx(g, dim, cell[0], cell[1], cell[2]);
+    if (guard) atomicAdd(&d.cnt[3], (unsigned long long)guard);
+  }
+  // phase 3: the two sides of a face are neighbouring lanes; each takes the
+  // other's flux and conserved vector and writes the LAXF flux of half the
+  // slots (side 0: [0, 7), side 1: [7, 13))
+  double fo[NV], uo[NV];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    fo[iv] = __shfl_xor(fl[iv], 1);
+    uo[iv] = __shfl_xor(ul[iv], 1);
+  }
+  if (!act) return;
   const long nfd = g.nface[dim];
-  double *fo = d.flb[dim];
-  const double ag = sh.ag[fs], agr = sh.agr[fs];
-  const int iv0 = j < 64 ? 0 : 7, iv1 = j < 64 ? 7 : NV;
-  for (int iv = iv0; iv < iv1; ++iv)
-    fo[iv * nfd + fi] = laxf(sh.f[0][iv][fs], sh.f[1][iv][fs], sh.u[0][iv][fs], sh.u[1][iv][fs], iv >= EE ? agr : ag);
+  double *out = d.flb[dim];
+#pragma unroll
+  for (int iv = 0; iv < NV; ++iv) {
+    if ((iv < 7) != (side == 0)) continue;
+    const double fL = side == 0 ? fl[iv] : fo[iv], fR = side == 0 ? fo[iv] : fl[iv];
+    const double uL = side == 0 ? ul[iv] : uo[iv], uR = side == 0 ? uo[iv] : ul[iv];
+    out[iv * nfd + fi] = laxf(fL, fR, uL, uR, iv >= EE ? agr : ag);
+  }
+}
+// One launch for the faces of every active dimension (faces of different
+// dimensions are independent): blocks [0, nb[0]) are x tiles, then y, then
+// z.  VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
+template <bool VISC>
+__global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, Dev d, kz_physics ph, FaceTiles ft) {
+  __shared__ FaceShared sh;
+  long b = blockIdx.x;
+  if (b < ft.nb[0]) face_tile<0, VISC>(g, d, ph, ft, b, sh);
+  else if (b < ft.nb[0] + ft.nb[1]) face_tile<1, VISC>(g, d, ph, ft, b - ft.nb[0], sh);
+  else face_tile<2, VISC>(g, d, ph, ft, b - ft.nb[0] - ft.nb[1], sh);
 }
 
 
@@ -2553,11 +2617,10 @@ static long ring_count(const GridD &g) {
 static void launch_set_bc(kz_sim *s) {
   const GridD &g = s->g;
   TimedLaunch tl(s, KZ_K_BC);
-  long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2 + (long)g.ny * g.nx * g.ngz * 2;
+  long nb = (long)g.nz * g.ny * g.ngx * 2 + (long)g.nz * g.nx * g.ngy * 2 + (long)g.ny * g.nx * g.ngz * 2 +
+            (g.nz == 1 ? 4L * (2 * (g.ng - 1) + 2) : 0);  // + the 2D corner cells
   k_bc<<<nblk(nb, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
-  if (g.nz == 1) {
-    k_bc_corners<<<1, 64, 0, s->stream>>>(g, s->d, s->ph);
-  } else {
+  if (g.nz > 1) {
     const long nslice = 4L * (g.nx + g.ny + g.nz), per = 2L * (g.ng - 1) + 1;
     k_bc_edges1<<<nblk(nslice * per, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
     k_bc_edges2<<<nblk(nslice + 8L * g.ng * g.ng * g.ng, 64), 64, 0, s->stream>>>(g, s->d, s->ph);
@@ -2585,6 +2648,26 @@ static void launch_calc_u2p(kz_sim *s, bool with_bc) {
   if (with_bc) launch_set_bc(s);
 }
 
+// The face-flux blocks of each dimension (k_face_flux): x faces in runs of
+// fx whose adjacent cells, over at most ceil((fx - 1) / (nx + 1)) row
+// changes, number at most 64; y and z faces in FT_T x FT_F tiles over the
+// cross extent (ghost ring included) and the nd + 1 faces along the dimension.
+static FaceTiles face_tiles(const GridD &g) {
+  FaceTiles ft{};
+  const long nd3[3] = {g.nx, g.ny, g.nz};
+  const long wf0 = g.nx + 1;
+  int fx = 62;
+  while (fx > 1 && fx + 1 + (fx - 1 + wf0 - 1) / wf0 > 64) --fx;
+  ft.fx = fx;
+  for (int dim = 0; dim < 3; ++dim) {
+    if (nd3[dim] <= 1) continue;
+    const long n0 = dim == 0 ? g.ny : g.nx, n1 = dim == 2 ? g.ny : g.nz;
+    const long w0 = n0 > 1 ? n0 + 2 : n0, w1 = n1 > 1 ? n1 + 2 : n1, wf = nd3[dim] + 1;
+    if (dim == 0) ft.nb[0] = (wf * w0 * w1 + fx - 1) / fx;
+    else ft.nb[dim] = ((w0 + FT_T - 1) / FT_T) * ((wf + FT_F - 1) / FT_F) * w1;
+  }
+  return ft;
+}
 static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
   const GridD &g = s->g;
   long nw = ring_count(g);
@@ -2593,16 +2676,11 @@ static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
     k_wavespeeds<<<nblk(nw, 128), 128, 0, s->stream>>>(g, s->d, s->ph);
   }
   {
-    long nf = 0;  // face_count over the three dimensions
-    const long nd3[3] = {g.nx, g.ny, g.nz};
-    for (int dim = 0; dim < 3; ++dim) {
-      if (nd3[dim] <= 1) continue;
-      const long n0 = dim == 0 ? g.ny : g.nx, n1 = dim == 2 ? g.ny : g.nz;
-      nf += (nd3[dim] + 1) * (n0 > 1 ? n0 + 2 : n0) * (n1 > 1 ? n1 + 2 : n1);
-    }
+    const FaceTiles ft = face_tiles(g);
+    const unsigned nb = (unsigned)(ft.nb[0] + ft.nb[1] + ft.nb[2]);
     TimedLaunch tl(s, KZ_K_FACE_FLUX);
-    if (s->ph.radviscosity) k_face_flux<true><<<nblk(nf, 64), 128, 0, s->stream>>>(g, s->d, s->ph);
-    else k_face_flux<false><<<nblk(nf, 64), 128, 0, s->stream>>>(g, s->d, s->ph);
+    if (s->ph.radviscosity) k_face_flux<true><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
+    else k_face_flux<false><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
   }
   long nc = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   {

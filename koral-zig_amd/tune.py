@@ -4,20 +4,22 @@ Each heavy kernel carries an `amdgpu_waves_per_eu` bound (KZ_WPE_* in
 csrc/kz_step.hip): a larger bound caps the registers per lane so that more
 waves share a SIMD and hide each other's latency, at the price of spills.
 The best bound depends on the kernel, the grid and the compiler, so it is
-measured; so is the width of the persistent implicit launches (blocks of
-one wave per CU, KZ_PERSIST_PER_CU), on the independent-cell batch: this module builds one library per candidate bound (only the
-bound of one kernel class differs from the defaults in each), times every
-kernel class of the PUFFY step on the GPU with HIP events in a subprocess
-per library, requires every candidate to leave the state bit for bit where
-the default build leaves it, and reports the fastest bound per class (the
-default unless another is at least 1 % faster).
+measured; so are the x width of the face flux kernel's y/z face tiles
+(KZ_FT_T) and the width of the persistent implicit launches (blocks of
+one wave per CU, KZ_PERSIST_PER_CU), the latter on the independent-cell
+batch.  This module builds one library per candidate (only one knob
+differs from the defaults in each), times every kernel class of the PUFFY
+step on the GPU with HIP events in a subprocess per library, requires every
+candidate to leave the state bit for bit where the default build leaves it,
+and reports the fastest value per knob (the default unless another is at
+least 1 % faster).
 
     python koral-zig_amd/tune.py [--jobs N] [--nx 384 --ny 360] [--out DIR]
 
 from the repository root, on a machine with the GPU.  The two halves can
 also run apart: `--build-only` compiles the libraries into --out (on any
 machine with hipcc), `--prebuilt` measures the libraries found there.  The
-defaults in kz_step.hip are the bounds this tuner selected.
+defaults in kz_step.hip are the values this tuner selected.
 """
 import argparse
 import concurrent.futures as cf
@@ -32,15 +34,17 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
 
-# kernel class (hip.KERNELS name) -> the bound macro that governs it, with the
-# default and the candidates tried
+# macro -> the kernel class (hip.KERNELS name) it governs, its default and
+# the candidates tried: the waves-per-EU bounds, and the x width of the face
+# flux kernel's y/z face tiles (KZ_FT_T columns by 64 / KZ_FT_T cells)
 KNOBS = {
-    "face_flux": ("KZ_WPE_FACE", 2, (1, 2, 3)),
-    "update": ("KZ_WPE_UPDATE", 2, (1, 2, 3)),
-    "u2p": ("KZ_WPE_U2P", 2, (1, 2, 3)),
-    "wavespeeds": ("KZ_WPE_WS", 1, (1, 2, 4)),
-    "implicit": ("KZ_WPE_PREP", 2, (1, 2, 3)),
-    "implicit_rung0": ("KZ_WPE_RUNG0", 1, (1, 2)),
+    "KZ_WPE_FACE": ("face_flux", 2, (1, 2, 3)),
+    "KZ_FT_T": ("face_flux", 8, (4, 8, 16)),
+    "KZ_WPE_UPDATE": ("update", 2, (1, 2, 3)),
+    "KZ_WPE_U2P": ("u2p", 2, (1, 2, 3)),
+    "KZ_WPE_WS": ("wavespeeds", 1, (1, 2, 4)),
+    "KZ_WPE_PREP": ("implicit", 2, (1, 2, 3)),
+    "KZ_WPE_RUNG0": ("implicit", 1, (1, 2)),
 }
 CLASSES = ("face_flux", "implicit", "u2p", "update", "wavespeeds", "ct", "bc")
 # the persistent implicit launches' width (blocks of one wave per CU): timed
@@ -51,7 +55,7 @@ BATCH_KNOBS = {"KZ_PERSIST_PER_CU": (4, (2, 4, 8))}
 def variants():
     """(name, {macro: value}) for the default build and each single change."""
     out = [("default", {})]
-    for cls, (macro, default, cands) in KNOBS.items():
+    for macro, (cls, default, cands) in KNOBS.items():
         for v in cands:
             if v != default:
                 out.append((f"{macro}={v}", {macro: v}))
@@ -202,8 +206,7 @@ def main():
         raise RuntimeError(f"variants changed the state bits: {bad}")
     print("every variant leaves the state bit for bit as the default build")
     pick = {}
-    for cls, (macro, default, cands) in KNOBS.items():
-        col = "implicit" if cls.startswith("implicit") else cls
+    for macro, (col, default, cands) in KNOBS.items():
         times = {default: res["default"]["per"][col]}
         for v in cands:
             if v != default:
