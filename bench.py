@@ -371,6 +371,7 @@ def main():
     ap.add_argument("--cpu-cells", type=int, default=20000, help="implicit-batch: oracle sample size (0: skip)")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="implicit-batch: skip the FP32-Jacobian comparison")
+    ap.add_argument("--lib", help="load this libkz_hip.so instead of the in-tree one (A/B of two builds)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs the step on one GPU only; --gpus must be 1 "
@@ -378,6 +379,9 @@ def main():
         sys.exit(2)
 
     _pkg()
+    if args.lib:
+        from koral_zig_amd import hip
+        hip.LIB_PATH = Path(args.lib).resolve()
     if args.workload == "implicit-batch":
         if "--steps" not in sys.argv:
             args.steps = 5
