@@ -1379,22 +1379,35 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(S s, ImpWor
   if (t >= nd) return;
   if (s.skip(ph, t)) return;
   double dt = s.dt(t);
-  double pp00[NV], uu00[NV], pp0[NV];
+  double pp00[NV], pp0[NV];
   s.load_p(t, pp00);
   Geo gm;
   s.geo(t, gm);
-  p2u(pp00, uu00, gm, ph.gam);
-  double ehat = -calc_ff_rtt(pp00, gm);
-  RadState st00;
-  fill_radstate(pp00, gm, ph.gam, ph.opac, true, st00);
-  ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
-  if (ph.implicit.start_with_bisect) {
-    solve_1d_prim(pp00, st00.ehat, st00.ucon[0], st00.tgas, dt, cx, pp0);
-  } else {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
+  // the reference state and the starting block are stored before the
+  // bisection, so that only its three inputs stay live through it
+  double st_ehat, st_ut, st_tgas;
+  int sw;
+  {
+    double uu00[NV];
+    p2u(pp00, uu00, gm, ph.gam);
+    st(w.uu00, nd, t, uu00);
+    const double ehat = -calc_ff_rtt(pp00, gm);
+    RadState st00;
+    fill_radstate(pp00, gm, ph.gam, ph.opac, true, st00);
+    sw = (ehat < ph.implicit.threshold * pp00[UU]) ? WP_RAD : WP_MHD;
+    double *a = w.aux;
+    a[IA_EHAT * nd + t] = st00.ehat;
+    a[IA_UINT * nd + t] = st00.uint;
+    a[IA_SGAS * nd + t] = st00.sgas;
+    a[IA_KES * nd + t] = st00.kappaes;
+    a[IA_KGA * nd + t] = st00.op.gas_abs;
+    a[IA_KRA * nd + t] = st00.op.rad_abs;
+    a[IA_KRR * nd + t] = st00.op.rad_ross;
+    a[IA_SW * nd + t] = (double)sw;
+    st_ehat = st00.ehat;
+    st_ut = st00.ucon[0];
+    st_tgas = st00.tgas;
   }
-  int sw = (ehat < ph.implicit.threshold * pp00[UU]) ? WP_RAD : WP_MHD;
   // gather the cell into the list of its starting block, so that each
   // first-rung kernel runs full waves of one Newton variant
   if (sw == WP_MHD) {
@@ -1404,17 +1417,14 @@ __global__ void __launch_bounds__(64) KZ_WPE(KZ_WPE_PREP) k_imp_prep(S s, ImpWor
     unsigned long long k = atomicAdd(&w.ncnt[1], 1ULL);
     w.list[nd + k] = (int32_t)t;
   }
-  st(w.uu00, nd, t, uu00);
+  ImpCtx cx{&gm, ph.gam, &ph.rad, &ph.opac, &ph.implicit, 1.0};
+  if (ph.implicit.start_with_bisect) {
+    solve_1d_prim(pp00, st_ehat, st_ut, st_tgas, dt, cx, pp0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) pp0[i] = pp00[i];
+  }
   st(w.pp0, nd, t, pp0);
-  double *a = w.aux;
-  a[IA_EHAT * nd + t] = st00.ehat;
-  a[IA_UINT * nd + t] = st00.uint;
-  a[IA_SGAS * nd + t] = st00.sgas;
-  a[IA_KES * nd + t] = st00.kappaes;
-  a[IA_KGA * nd + t] = st00.op.gas_abs;
-  a[IA_KRA * nd + t] = st00.op.rad_abs;
-  a[IA_KRR * nd + t] = st00.op.rad_ross;
-  a[IA_SW * nd + t] = (double)sw;
 }
 __device__ __forceinline__ void imp_load(const ImpWork &w, long nd, long t, double *uu00, double *pp0, State0 &s0,
                                          LagOpac &lag) {
