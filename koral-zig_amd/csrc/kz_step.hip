@@ -48,7 +48,7 @@ struct GridD {
 };
 
 struct Dev {
-  double *p, *u, *ut0, *ut1, *ut2, *dut1, *dut2, *drt1, *drt2, *uforget;
+  double *p, *u, *ut0, *ut1, *ut2, *dut1, *drt1, *drt2, *uforget;
   double *flb[3];
   double *scal;  // planes: ahd_m[3], arad_m[3], tstepden
   int32_t *flags;
@@ -364,33 +364,28 @@ __device__ void bc_corner2d_cell(const GridD &g, const Dev &d, const kz_physics 
   const long xin = hx ? nx - 1 : 0, yin = hy ? ny - 1 : 0;  // edge domain index
   const long xg1 = hx ? nx : -1, yg1 = hy ? ny : -1;        // first ghost index
   const long xg2 = hx ? nx + 1 : -2, yg2 = hy ? ny + 1 : -2;
+  // the corner cell: copies first (k < 2 (ng - 1)), then the inner and the
+  // outer diagonal
+  const bool copy = k < 2 * (ng - 1), inner = k == 2 * (ng - 1);
+  const long i = k / 2;
+  const long dx_ = copy ? (k % 2 == 0 ? (hx ? nx + i + 1 : -ng + i) : xg1) : (inner ? xg1 : xg2);
+  const long dy_ = copy ? (k % 2 == 0 ? yg1 : (hy ? ny + i + 1 : -ng + i)) : (inner ? yg1 : yg2);
+  Geo gm;  // the corner cell's geometry, loaded before the ghost values are formed
+  geo_c(g, d, dx_, dy_, 0, gm);
   double pp[NV];
-  long dx_, dy_;
-  if (k < 2 * (ng - 1)) {
-    const long i = k / 2;
-    if (k % 2 == 0) {  // (x, yg1) <- x ghost (x, yin)
-      dx_ = hx ? nx + i + 1 : -ng + i;
-      dy_ = yg1;
-      corner_x_src(g, d, dx_, yin, pp);
-    } else {  // (xg1, y) <- y ghost (xin, y)
-      dx_ = xg1;
-      dy_ = hy ? ny + i + 1 : -ng + i;
-      bc_y_prims(g, d, xin, dy_, 0, pp);
-    }
+  if (copy && k % 2 == 0) {  // (x, yg1) <- x ghost (x, yin)
+    corner_x_src(g, d, dx_, yin, pp);
+  } else if (copy) {  // (xg1, y) <- y ghost (xin, y)
+    bc_y_prims(g, d, xin, dy_, 0, pp);
   } else {
     // inner: x ghost (xg1, yin) and y ghost (xin, yg1); outer: the copies
     // (xg2, yg1) of x ghost (xg2, yin) and (xg1, yg2) of y ghost (xin, yg2)
-    const bool inner = k == 2 * (ng - 1);
-    dx_ = inner ? xg1 : xg2;
-    dy_ = inner ? yg1 : yg2;
     double pa[NV], pb[NV];
     corner_x_src(g, d, dx_, yin, pa);
     bc_y_prims(g, d, xin, dy_, 0, pb);
 #pragma unroll
     for (int iv = 0; iv < NV; ++iv) pp[iv] = 0.5 * (pa[iv] + pb[iv]);
   }
-  Geo gm;
-  geo_c(g, d, dx_, dy_, 0, gm);
   store_p2u(g, d, ph.gam, dx_, dy_, 0, gm, pp);
 }
 
@@ -2178,7 +2173,8 @@ __global__ void k_stage_explicit1(GridD g, Dev d) {
   }
 }
 // after the second explicit stage: dut2, u = ut0 + dt/2 dut1 + dt/2 dut2,
-// then u += dt/2 drt1 + dt/2 drt2
+// then u += dt/2 drt1 + dt/2 drt2.  dut2 is read only here, so it stays in
+// a register (the reference keeps it as a field, sim.zig:1877-1880).
 __global__ void k_stage_explicit2(GridD g, Dev d) {
   long t = TID, c;
   if (!dom_cell(g, t, c)) return;
@@ -2188,7 +2184,6 @@ __global__ void k_stage_explicit2(GridD g, Dev d) {
   for (int iv = 0; iv < NV; ++iv) {
     long o = iv * n + c;
     double du2 = f1 * d.u[o] + f2 * d.ut2[o];
-    d.dut2[o] = du2;
     double u = d.ut0[o] + h * d.dut1[o] + h * du2;
     d.u[o] = u + h * d.drt1[o] + h * d.drt2[o];
   }
@@ -2457,7 +2452,7 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) { *err = KZ_E_HIP; delete s; return nullptr; }
   size_t fb = (size_t)G.ncell * NV * sizeof(double);
   Dev &D = s->d;
-  double **fl[] = {&D.p, &D.u, &D.ut0, &D.ut1, &D.ut2, &D.dut1, &D.dut2, &D.drt1, &D.drt2, &D.uforget, &D.stage};
+  double **fl[] = {&D.p, &D.u, &D.ut0, &D.ut1, &D.ut2, &D.dut1, &D.drt1, &D.drt2, &D.uforget, &D.stage};
   int e = 0;
   for (auto f : fl) first_err(e, dalloc(s, (void **)f, fb));
   for (int d = 0; d < 3; ++d) first_err(e, dalloc(s, (void **)&D.flb[d], (size_t)G.nface[d] * NV * sizeof(double)));
