@@ -1062,13 +1062,14 @@ __global__ void k_emf(GridD g, Dev d) {
 __device__ __forceinline__ double emf_at(const GridD &g, const Dev &d, int c, long ix, long iy, long iz) {
   return d.emf[c][eidx(g, ix, iy, iz)];
 }
-// Two threads per cell: side 0 forms the metric source terms, side 1 the
-// flux differences; both leave their 13 rows in LDS and the update
-// u + du + ms dt is then written half by each (the same expression as one
-// thread evaluating both).
+// Two threads per cell, in different waves: for the block's 64 cells, wave 0
+// forms the metric source terms and wave 1 the flux differences, so each
+// wave runs one of the two paths with all its lanes; both leave their 13
+// rows in LDS and the update u + du + ms dt is then written half by each
+// wave (the same expression as one thread evaluating both).
 __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_UPDATE) k_update(GridD g, Dev d, kz_physics ph, int sslot) {
   __shared__ double sms[NV][64], sdu[NV][64];
-  const int slot = threadIdx.x >> 1, side = threadIdx.x & 1;
+  const int slot = threadIdx.x & 63, side = threadIdx.x >> 6;
   const long t = (long)blockIdx.x * 64 + slot;
   const bool live = t < (long)g.nx * g.ny * g.nz;
   long ix = 0, iy = 0, iz = 0, c = 0;
