@@ -6,8 +6,10 @@
  * device code the step kernels use.  Geometry arrives as kz_geom records
  * (koral/geometry.zig:29-49), states as 13 doubles per record in the slot
  * order of kz_types.h.  Buffers are host memory; every call is synchronous.
- * Return: KZ_OK, KZ_E_HIP on a runtime failure, KZ_E_INVALID_CONFIG /
- * KZ_E_UNSUPPORTED for arguments outside this build.
+ * Return: KZ_OK, KZ_E_HIP on a runtime failure, KZ_E_INVALID_CONFIG for
+ * arguments outside the reference's ranges (an inversion type, dimension or
+ * function index out of range, a b^2/rho floor frame other than drift or
+ * ZAMO).
  */
 #ifndef KZ_CELLOPS_H
 #define KZ_CELLOPS_H

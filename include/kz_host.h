@@ -59,6 +59,10 @@ int kz_host_calc_b_from_a(const kz_grid *g, const double *gc, double *p);
 double kz_host_uniform(uint64_t seed, uint64_t cell, uint64_t var);
 int kz_host_perturb(const kz_grid *g, uint64_t seed, double amp, double gam, int rho_u_only, double *p);
 
+/* sim.zig:481-495: the largest radial index (ghosts included) whose cell
+ * centre lies inside the BL horizon, kz_physics.reduce_order_ix_max for
+ * reduceorderatbh; INT32_MIN when there is none */
+int32_t kz_host_reduce_order_ix_max(const kz_grid *g, double a, double mksr0, double mksh0);
 /* BL radius of each radial index (scalars.zig:47-50 radiusBL), r[nx] */
 int kz_host_radius_bl(const kz_grid *g, double a, double mksr0, double mksh0, double *r);
 /* BL theta and phi widths of the cells of radial shell ix, [nz][ny] each

@@ -27,11 +27,11 @@ typedef struct kz_sim kz_sim;
 /* Sim.init (koral/sim.zig:309-500) with the MetricCache blocks computed by
  * the caller (kz_host_metric_tables) and the PUFFY outer-boundary table
  * (kz_host_bc_xhi_tables). Returns NULL and sets *err on failure:
- * KZ_E_UNSUPPORTED for a grid with a collapsed x or y dimension, fewer than
- * 3 ghosts, or an option outside this build (reduce-order, synchrotron
- * bridge, ZAMO floors), KZ_E_INVALID_CONFIG for
- * sim.zig:319-363's preconditions (including radiative viscosity without
- * opacities).  With radviscosity or dynamo set, the BL sidecar must be
+ * KZ_E_UNSUPPORTED for a grid with a collapsed x or y dimension or fewer
+ * than 3 ghosts, KZ_E_INVALID_CONFIG for sim.zig:319-363's preconditions
+ * (including radiative viscosity without opacities) and for a b^2/rho floor
+ * frame other than drift (0) or ZAMO (1), the two values of
+ * koral/solve/invert.zig:35.  With radviscosity or dynamo set, the BL sidecar must be
  * loaded with kz_sim_set_bl_tables before the first step. */
 kz_sim *kz_sim_create(const kz_grid *grid, const kz_physics *phys, const double *g, const double *gcon,
                       const double *kris, const double *gb0, const double *gconb0, const double *gb1,

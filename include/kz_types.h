@@ -37,7 +37,7 @@ enum {
   KZ_E_OOM = -4,
   KZ_E_INVALID_CONFIG = -5,  /* Sim.init preconditions (koral/sim.zig:319-363) */
   KZ_E_HIP = -6,             /* a HIP runtime call failed */
-  KZ_E_UNSUPPORTED = -7,     /* a reference option this build does not implement */
+  KZ_E_UNSUPPORTED = -7,     /* a grid this build does not run: a collapsed x or y dimension, < 3 ghosts */
   KZ_E_BAD_MAGIC = -8,       /* dump.ReadError.BadMagic (koral/io/dump.zig:43) */
   KZ_E_BAD_VERSION = -9,     /* dump.ReadError.BadVersion */
   KZ_E_DIM_MISMATCH = -10,   /* dump.ReadError.DimMismatch */
@@ -56,7 +56,7 @@ typedef struct kz_geom {
 } kz_geom;
 
 /* MHD floors (koral/solve/invert.zig:47-89). b2rhofloorframe: 0 = drift
- * frame (the validated build); 1 = ZAMO frame (not implemented here). */
+ * frame (the validated build); 1 = ZAMO frame (invert.zig:649-684). */
 typedef struct kz_floor_params {
   double rhofloor;
   double uurhoratiomin;
@@ -130,7 +130,7 @@ typedef struct kz_opac_params {
   double kappaes_grey;         /* kappaes_mode == 2: kappaes = kappaes_grey * rho */
   double opdamp;               /* four-force scale (1.0 = undamped) */
   int32_t bremsstrahlung, synchrotron, kleinnishina, comptonization;
-  int32_t synchrotron_bridge;  /* must be 0 (not implemented) */
+  int32_t synchrotron_bridge;  /* Ramesh NR bridge instead of the Terelfactor (opacities.zig:70-74) */
   int32_t kappa_mode;          /* 0 default (opacities), 1 grey */
   int32_t kappaes_mode;        /* 0 none, 1 PUFFY Klein-Nishina, 2 grey */
   int32_t _pad;
@@ -165,12 +165,16 @@ typedef struct kz_physics {
   int32_t do_radimp_fixups;
   int32_t correct_polaraxis;
   int32_t nccorrectpolar;
-  int32_t reduceorderatbh;        /* must be 0 */
-  int32_t reduceorderafterfixup;  /* must be 0 */
+  int32_t reduceorderatbh;        /* cells with radial index <= reduce_order_ix_max reconstruct
+                                     one order lower (sim.zig:1316-1320) */
+  int32_t reduceorderafterfixup;  /* cells with a fixup flag set reconstruct one order lower
+                                     (sim.zig:1321-1332) */
   int32_t dampradwavespeednearaxis;
   int32_t radviscosity;           /* radiative shear viscosity (needs the BL tables) */
   int32_t dynamo;                 /* mean-field dynamo after each explicit stage (BL tables) */
-  int32_t _pad;
+  int32_t reduce_order_ix_max;    /* the largest radial index, ghosts included, whose centre lies
+                                     inside the BL horizon (sim.zig:481-495); read only with
+                                     reduceorderatbh; kz_host_reduce_order_ix_max computes it */
   kz_radvisc_params radvisc;
   kz_dynamo_params dynamo_params;
   double bh_spin;                 /* metric spin a (Options.mp.a) */

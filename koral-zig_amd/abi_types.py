@@ -89,7 +89,7 @@ class kz_physics(C.Structure):
         [(n, C.c_int32) for n in (
             "has_opac", "do_fixups", "do_u2prad_fixups", "do_radimp_fixups", "correct_polaraxis",
             "nccorrectpolar", "reduceorderatbh", "reduceorderafterfixup", "dampradwavespeednearaxis",
-            "radviscosity", "dynamo", "_pad")] + \
+            "radviscosity", "dynamo", "reduce_order_ix_max")] + \
         [("radvisc", kz_radvisc_params), ("dynamo_params", kz_dynamo_params), ("bh_spin", C.c_double)]
 
 

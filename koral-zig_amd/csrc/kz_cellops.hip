@@ -65,6 +65,7 @@ __global__ void b_u2p_mhd(long n, const double *uu, double *pp, const kz_geom *g
   r[i] = o;
 }
 
+template <bool ZAMO>
 __global__ void b_check_floors_mhd(long n, double *pp, const kz_geom *g, double gam, kz_floor_params f, int32_t *ret,
                                    unsigned long long *nguard) {
   long i = TID;
@@ -74,7 +75,7 @@ __global__ void b_check_floors_mhd(long n, double *pp, const kz_geom *g, double 
   double p[NV];
   for (int k = 0; k < NV; ++k) p[k] = pp[i * NV + k];
   int guard = 0;
-  ret[i] = check_floors_mhd(p, gm, gam, f, &guard);
+  ret[i] = check_floors_mhd<ZAMO>(p, gm, gam, f, &guard);
   for (int k = 0; k < NV; ++k) pp[i * NV + k] = p[k];
   if (guard) atomicAdd(nguard, (unsigned long long)guard);
 }
@@ -329,7 +330,9 @@ int kz_check_floors_mhd_batch(int64_t n, double *pp, const kz_geom *g, double ga
   int32_t *dr = s.alloc<int32_t>(n);
   unsigned long long *dn = s.alloc<unsigned long long>(1);
   if (s.bad) return KZ_E_HIP;
-  b_check_floors_mhd<<<nb(n, 64), 64>>>(n, dp, dg, gam, *f, dr, dn);
+  if (f->b2rhofloorframe < 0 || f->b2rhofloorframe > 1) return KZ_E_INVALID_CONFIG;
+  if (f->b2rhofloorframe == 1) b_check_floors_mhd<true><<<nb(n, 64), 64>>>(n, dp, dg, gam, *f, dr, dn);
+  else b_check_floors_mhd<false><<<nb(n, 64), 64>>>(n, dp, dg, gam, *f, dr, dn);
   if (int e = s.finish_launch()) return e;
   s.out(pp, dp, n * NV);
   s.out(ret, dr, n);
@@ -372,7 +375,6 @@ int kz_solve_implicit_lab_batch(int64_t n, double *uu, double *pp, const kz_geom
                                 const kz_rad_params *rp, const kz_opac_params *op, const kz_implicit_params *ip,
                                 kz_implicit_result *r) {
   if (n <= 0) return 0;
-  if (op->synchrotron_bridge) return KZ_E_UNSUPPORTED;
   Stage s;
   double *du = s.in(uu, n * NV), *dp = s.in(pp, n * NV), *ddt = s.in(dt, n);
   kz_geom *dg = s.in(g, n);
@@ -475,7 +477,6 @@ int kz_rij_batch(int64_t n, const double *pp, const kz_geom *g, double *out21) {
 int kz_radstate_batch(int64_t n, const double *pp, const kz_geom *g, double gam, const kz_opac_params *op,
                       int32_t full, double *out10) {
   if (n <= 0) return 0;
-  if (op->synchrotron_bridge) return KZ_E_UNSUPPORTED;
   Stage s;
   double *dp = s.in(pp, n * NV), *dout = s.alloc<double>(n * 10);
   kz_geom *dg = s.in(g, n);

@@ -206,10 +206,17 @@ KZD int u2p_mhd(const double *uu_in, double *pp, const Geo &gm, double gam, cons
   return ret;
 }
 
-// invert.zig:474-715 checkFloorsMhd, drift-frame branch; *guard counts
-// the guard-ladder recoveries (n_guard_recoveries)
+// invert.zig:474-715 checkFloorsMhd; *guard counts the guard-ladder
+// recoveries (n_guard_recoveries).  ZAMO selects the b^2/rho floor in the
+// ZAMO frame (b2rhofloorframe 1) at compile time, so that the kernels of the
+// default drift frame carry none of its code.
+template <bool ZAMO = false>
 KZD int check_floors_mhd(double *pp, const Geo &gm, double pg, const kz_floor_params &fp, int *guard) {
   int ret = 0;
+  // invert.zig:485-492: the ZAMO floor adds its delta to the conserveds of
+  // the primitives as they enter, before any floor
+  double uu_entry[NV];
+  if (ZAMO) p2u_mhd(pp, uu_entry, gm, pg);
   if (pp[RHO] < fp.rhofloor) {
     pp[RHO] = fp.rhofloor;
     ret = -1;
@@ -233,11 +240,44 @@ KZD int check_floors_mhd(double *pp, const Geo &gm, double pg, const kz_floor_pa
       f = bsq / (fp.b2rhoratiomax * pp[RHO]);
       floored = true;
     }
-    if (fp.b2rhofloorframe == 0 && fp.b2uufloor && bsq > fp.b2uuratiomax * pp[UU]) {
+    if (!ZAMO && fp.b2rhofloorframe == 0 && fp.b2uufloor && bsq > fp.b2uuratiomax * pp[UU]) {
       fuu = bsq / (fp.b2uuratiomax * pp[UU]);
       floored = true;
     }
-    if (floored) {
+    if (ZAMO && floored) {
+      // invert.zig:649-684: the new mass drho = rho (f - 1) moves with the
+      // normal observer n^mu = -alpha g^{mu 0}; its conserved vector is added
+      // to the entry conserveds through B3 and the primitives recovered by the
+      // hot, then the entropy solver; if both fail the fluid-frame backup
+      // scales rho and u by f
+      double pporg[NV];
+#pragma unroll
+      for (int iv = 0; iv < NV; ++iv) pporg[iv] = pp[iv];
+      const double drho = pp[RHO] * (f - 1.0);
+      double etacon[4], etarel[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) etacon[i] = -gm.alpha * gm.G[i][0];
+      vel4_to_velr(etacon, gm, true, etarel);
+      double dpp[NV], duu[NV], uu[NV];
+#pragma unroll
+      for (int iv = 0; iv < NV; ++iv) dpp[iv] = 0.0;
+      dpp[RHO] = drho;
+      dpp[VX] = etarel[1];
+      dpp[VY] = etarel[2];
+      dpp[VZ] = etarel[3];
+      p2u_mhd(dpp, duu, gm, pg);
+#pragma unroll
+      for (int iv = 0; iv < NV; ++iv) uu[iv] = iv <= B3 ? uu_entry[iv] + duu[iv] : 0.0;
+      int rt = u2p_solver_w(uu, pp, gm, 0, pg);
+      if (rt < 0) rt = u2p_solver_w(uu, pp, gm, 1, pg);
+      if (rt < 0) {
+#pragma unroll
+        for (int iv = 0; iv < NV; ++iv) pp[iv] = pporg[iv];
+        pp[RHO] *= f;
+        pp[UU] *= f;
+      }
+      ret = -1;
+    } else if (floored) {
       double wold = pp[RHO] + pp[UU] * pg;
       pp[RHO] *= f;
       pp[UU] *= fp.isentropic_b2rhofloors ? f : fuu;

@@ -605,6 +605,7 @@ __device__ void fixup_cell(const GridD &g, const Dev &d, const kz_physics &ph, l
 
 // u2p pass (sim.zig:940-1000); cells flagged for the u2p fixups
 // (sim.zig:1001-1003) are listed for k_u2p_fixups.
+template <bool ZAMO>
 __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_U2P) k_u2p(GridD g, Dev d, kz_physics ph) {
   long t = TID;
   long nd = (long)g.nx * g.ny * g.nz;
@@ -634,7 +635,7 @@ __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_U2P) k_u2p(GridD g, Dev d, 
       int corr;
       u2p_rad(uu, pp, gm, ph.rad, &corr);
       int guard = 0;
-      check_floors_mhd(pp, gm, ph.gam, ph.floors, &guard);
+      check_floors_mhd<ZAMO>(pp, gm, ph.gam, ph.floors, &guard);
       check_floors_rad(pp, gm, ph.rad);
       st(d.p, n, c, pp);
       int fh = ret < -1 ? 1 : 0, fr = corr ? -1 : 0;
@@ -872,7 +873,21 @@ struct FaceTile {
     return true;
   }
 };
-template <int DIM, bool VISC>
+// sim.zig:1316-1337: the reconstruction order of a cell: PPM, one lower with
+// REDUCEORDERATBH inside the horizon (radial index <= reduce_order_ix_max)
+// or REDUCEORDERAFTERFIXUP on a cell that the last u2p or implicit pass
+// flagged for a fixup (ghost cells carry no flag)
+__device__ __forceinline__ int recon_order(const GridD &g, const Dev &d, const kz_physics &ph, const long cell[3]) {
+  if (ph.reduceorderatbh && cell[0] <= ph.reduce_order_ix_max) return 1;
+  if (ph.reduceorderafterfixup) {
+    const long c = cidx(g, cell[0], cell[1], cell[2]), n = g.ncell;
+    if (d.flags[KZ_FLAG_HD_FIXUP * n + c] != 0 || d.flags[KZ_FLAG_RAD_FIXUP * n + c] != 0 ||
+        d.flags[KZ_FLAG_RADIMP_FIXUP * n + c] != 0)
+      return 1;
+  }
+  return 2;
+}
+template <int DIM, bool VISC, bool ZAMO>
 __device__ __forceinline__ void face_tile(const GridD &g, const Dev &d, const kz_physics &ph, const FaceTiles &ft,
                                           long b, FaceShared &sh) {
   constexpr int dim = DIM;
@@ -891,13 +906,15 @@ __device__ __forceinline__ void face_tile(const GridD &g, const Dev &d, const kz
 #pragma unroll
     for (int k = 0; k < 5; ++k) dxs[k] = csize(g, i - 2 + k, dim);
     const int iv0 = j < 64 ? 0 : 7, iv1 = j < 64 ? 7 : NV;
+    const int order = recon_order(g, d, ph, cell);
     for (int iv = iv0; iv < iv1; ++iv) {
       const double *f = d.p + iv * n;
       double s5[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) s5[k] = f[c + (k - 2) * stride];
       double lo, ro;
-      ppm(s5, dxs, &lo, &ro);
+      if (order == 2) ppm(s5, dxs, &lo, &ro);
+      else reconstruct(order, ph.minmod_theta, s5, dxs, &lo, &ro);
       sh.st[0][iv][kc] = lo;
       sh.st[1][iv][kc] = ro;
     }
@@ -918,7 +935,7 @@ __device__ __forceinline__ void face_tile(const GridD &g, const Dev &d, const kz
     Geo gm;
     geo_f(g, d, dim, cell[0], cell[1], cell[2], gm);
     int guard = 0;
-    check_floors_mhd(ps, gm, ph.gam, ph.floors, &guard);
+    check_floors_mhd<ZAMO>(ps, gm, ph.gam, ph.floors, &guard);
     if (flux_prime_d<DIM>(ps, gm, ph.gam, fl)) set_err(d.err, E_NANFLUX);
     p2u(ps, ul, gm, ph.gam);
     if (VISC) {
@@ -955,14 +972,15 @@ __device__ __forceinline__ void face_tile(const GridD &g, const Dev &d, const kz
 }
 // One launch for the faces of every active dimension (faces of different
 // dimensions are independent): blocks [0, nb[0]) are x tiles, then y, then
-// z.  VISC adds the radiative-viscosity flux (sim.zig:1349-1364).
-template <bool VISC>
+// z.  VISC adds the radiative-viscosity flux (sim.zig:1349-1364); ZAMO
+// floors in the ZAMO frame.
+template <bool VISC, bool ZAMO>
 __global__ void __launch_bounds__(128) KZ_WPE(KZ_WPE_FACE) k_face_flux(GridD g, Dev d, kz_physics ph, FaceTiles ft) {
   __shared__ FaceShared sh;
   long b = blockIdx.x;
-  if (b < ft.nb[0]) face_tile<0, VISC>(g, d, ph, ft, b, sh);
-  else if (b < ft.nb[0] + ft.nb[1]) face_tile<1, VISC>(g, d, ph, ft, b - ft.nb[0], sh);
-  else face_tile<2, VISC>(g, d, ph, ft, b - ft.nb[0] - ft.nb[1], sh);
+  if (b < ft.nb[0]) face_tile<0, VISC, ZAMO>(g, d, ph, ft, b, sh);
+  else if (b < ft.nb[0] + ft.nb[1]) face_tile<1, VISC, ZAMO>(g, d, ph, ft, b - ft.nb[0], sh);
+  else face_tile<2, VISC, ZAMO>(g, d, ph, ft, b - ft.nb[0] - ft.nb[1], sh);
 }
 
 
@@ -2411,8 +2429,10 @@ kz_sim *kz_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g, 
                       const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
   if (gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return nullptr; }
-  if (ph->reduceorderatbh || ph->reduceorderafterfixup || ph->opac.synchrotron_bridge ||
-      ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return nullptr; }
+  if (ph->floors.b2rhofloorframe < 0 || ph->floors.b2rhofloorframe > 1) {
+    *err = KZ_E_INVALID_CONFIG;
+    return nullptr;
+  }
   if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
   if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
   kz_sim *s = new kz_sim();
@@ -2648,7 +2668,8 @@ int kz_sim_set_bc(kz_sim *s, int ifinit) {
 static void launch_calc_u2p(kz_sim *s, bool with_bc) {
   {
     TimedLaunch tl(s, KZ_K_U2P);
-    k_u2p<<<nblk(ndom(s), 128), 128, 0, s->stream>>>(s->g, s->d, s->ph);
+    if (s->ph.floors.b2rhofloorframe == 1) k_u2p<true><<<nblk(ndom(s), 128), 128, 0, s->stream>>>(s->g, s->d, s->ph);
+    else k_u2p<false><<<nblk(ndom(s), 128), 128, 0, s->stream>>>(s->g, s->d, s->ph);
   }
   if (s->ph.do_fixups) k_u2p_fixups<<<1, 256, 0, s->stream>>>(s->g, s->d, s->ph);
   if (with_bc) launch_set_bc(s);
@@ -2685,8 +2706,11 @@ static void launch_op_explicit(kz_sim *s, int dt_slot, bool with_bc) {
     const FaceTiles ft = face_tiles(g);
     const unsigned nb = (unsigned)(ft.nb[0] + ft.nb[1] + ft.nb[2]);
     TimedLaunch tl(s, KZ_K_FACE_FLUX);
-    if (s->ph.radviscosity) k_face_flux<true><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
-    else k_face_flux<false><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
+    const bool zamo = s->ph.floors.b2rhofloorframe == 1;
+    if (s->ph.radviscosity && zamo) k_face_flux<true, true><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
+    else if (s->ph.radviscosity) k_face_flux<true, false><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
+    else if (zamo) k_face_flux<false, true><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
+    else k_face_flux<false, false><<<nb, 128, 0, s->stream>>>(g, s->d, s->ph, ft);
   }
   long nc = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
   {
@@ -3631,7 +3655,6 @@ kz_implicit_batch *kz_implicit_batch_create(int64_t n, const kz_physics *ph, int
   *err = 0;
   if (n <= 0 || n > INT32_MAX) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
   if (!ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return nullptr; }
-  if (ph->opac.synchrotron_bridge) { *err = KZ_E_UNSUPPORTED; return nullptr; }
   kz_implicit_batch *b = new kz_implicit_batch();
   b->n = n;
   b->ph = *ph;
@@ -3820,7 +3843,7 @@ extern "C" int kz_implicit_solve_noalloc(int64_t n, const double *pp_in, const k
             KZ_NOALLOC_MAX_CELLS);
     return KZ_E_INVALID_CONFIG;
   }
-  if (!phys || !phys->has_opac || phys->opac.synchrotron_bridge) return phys ? KZ_E_UNSUPPORTED : KZ_E_INVALID_CONFIG;
+  if (!phys || !phys->has_opac) return KZ_E_INVALID_CONFIG;
   if (n == 0) return KZ_OK;
   if (!pp_in || !g || !dt || !res) return KZ_E_INVALID_CONFIG;
   int dev = 0;

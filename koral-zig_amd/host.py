@@ -48,6 +48,8 @@ def lib():
         L.kz_host_bl_tables.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
         L.kz_host_radius_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, D]
         L.kz_host_shell_widths_bl.argtypes = [G, C.c_double, C.c_double, C.c_double, C.c_long, D, D]
+        L.kz_host_reduce_order_ix_max.restype = C.c_int32
+        L.kz_host_reduce_order_ix_max.argtypes = [G, C.c_double, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -181,6 +183,13 @@ def radius_bl(grid, mp=MP_PUFFY):
     r = np.zeros(grid.nx)
     lib().kz_host_radius_bl(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"], T.ptr(r))
     return r
+
+
+def reduce_order_ix_max(grid, mp=MP_PUFFY):
+    """sim.zig:481-495: the largest radial index (ghosts included) whose cell
+    centre lies inside the BL horizon, for kz_physics.reduce_order_ix_max
+    (reduceorderatbh); INT32_MIN when there is none."""
+    return int(lib().kz_host_reduce_order_ix_max(C.byref(grid), mp["a"], mp["mksr0"], mp["mksh0"]))
 
 
 def shell_widths_bl(grid, ix, mp=MP_PUFFY):

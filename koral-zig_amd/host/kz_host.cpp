@@ -1232,6 +1232,26 @@ int kz_host_radius_bl(const kz_grid *gr, double a, double mksr0, double mksh0, d
   return 0;
 }
 
+// sim.zig:481-495 (REDUCEORDERATBH): the BL radius of a cell centre grows
+// with the radial index alone, so the cells inside the horizon r_H(a) =
+// 1 + sqrt(1 - a^2) (metric.zig:20-23) form a prefix of the radial indices;
+// the largest index of that prefix over the full radial extent, ghosts
+// included, at the domain's middle theta row.  INT32_MIN when no centre is
+// inside.
+int32_t kz_host_reduce_order_ix_max(const kz_grid *gr, double a, double mksr0, double mksh0) {
+  Grid G(*gr);
+  MP mp{a, mksr0, mksh0};
+  const double r_h = 1.0 + std::sqrt(1.0 - a * a);
+  const double y_mid = G.yc(gr->ny / 2);
+  int32_t ix_max = INT32_MIN;
+  for (long jx = -gr->ng; jx < gr->nx + gr->ng; ++jx) {
+    double x[4] = {0.0, G.xc(jx), y_mid, 0.0}, o[4];
+    coco_mks2_to_bl(x, mp, o);
+    if (o[1] < r_h) ix_max = (int32_t)jx;
+  }
+  return ix_max;
+}
+
 // scalars.zig:188-204 blThetaWidth / blPhiWidth at radial index ix, for every
 // (iz, iy) of the shell: |theta_BL| across the two y faces at the x centre,
 // |phi_BL| across the two z faces at the cell centre; row-major [nz][ny]

@@ -27,6 +27,7 @@ int kzo_u2p_mhd_batch(int64_t n, const double *uu, double *pp, const kz_geom *g,
 int kzo_check_floors_mhd_batch(int64_t n, double *pp, const kz_geom *g, double gam, const kz_floor_params *f,
                                int32_t *ret, uint64_t *nguard) {
   uint64_t ng = 0;
+  if (f->b2rhofloorframe < 0 || f->b2rhofloorframe > 1) return KZ_E_INVALID_CONFIG;
   for (int64_t i = 0; i < n; ++i) { GEO(i); ret[i] = kzo_check_floors_mhd(pp + i * NV, &gm, gam, f, &ng); }
   if (nguard) *nguard = ng;
   return 0;

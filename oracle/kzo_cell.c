@@ -449,11 +449,11 @@ void kzo_calc_rad_wavespeeds(const double *pp, const kzo_geo *gm, const double t
 /* ------------------------------------------------------------------------ */
 /* opacities.zig                                                             */
 
-/* opacities.zig:131-315 calcOpacitiesFromStateG (bridge off, no MESA) */
+/* opacities.zig:131-315 calcOpacitiesFromStateG (no MESA), with the
+   synchrotron bridge of opacities.zig:208-297 when c->synchrotron_bridge */
 void kzo_calc_opacities(const kz_opac_params *c, double rho, double tgas, double te, double trad,
                         double tradbb, double ne, double bsq, int level, kzo_opac *o) {
   (void)tgas;
-  (void)tradbb;
   int want_rad = level != OPAC_CHI;
   int want_full = level == OPAC_FULL;
   double rt_te = kzm_sqrt(te);
@@ -487,7 +487,17 @@ void kzo_calc_opacities(const kz_opac_params *c, double rho, double tgas, double
   }
 
   if (c->synchrotron) {
+    /* the bridge clamps the synchrotron Trad up to TradBB^(4/3) / Te^(1/3),
+       adds the non-relativistic components to the gas and radiation
+       absorption, applies the number-opacity crossover at tc_n and drops the
+       Terelfactor suppression */
+    const int bridge = c->synchrotron_bridge != 0;
+    const double tc_n = 5.07783e9;
     double trad_syn = trad;
+    if (bridge) {
+      double trad_lim = kzm_pow(tradbb, 4.0 / 3.0) / kzm_pow(te, 1.0 / 3.0);
+      trad_syn = trad < trad_lim ? trad_lim : trad;
+    }
     double emis_synchro = 3.61e-34 * (nethcgs / rhocgs) * te * te * bmagcgs * bmagcgs;
     kappagassyn = c->kappacgs2gu * (emis_synchro / bbenergy) * rho;
     if (want_rad) {
@@ -500,11 +510,15 @@ void kzo_calc_opacities(const kz_opac_params *c, double rho, double tgas, double
       double ia_by_b_rad = bmagcgs * bmagcgs / zeta_a_denom_rad;
       double te5 = te * te * te * te * te;
       kapparadsyn = c->kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * ia_by_b_rad) * rho;
+      if (bridge)
+        kapparadsyn += c->kappacgs2gu *
+                       (2.35869e-21 * (nethcgs / rhocgs) * (bmagcgs * bmagcgs) / (trad_syn * trad_syn * trad_syn)) * rho;
       if (want_full) {
         double zeta_a_denom_num = (0.025 * kzm_cbrt(zbr * zbr * zbr * zbr * bmagcgs * bmagcgs) +
                                    0.169 * kzm_cbrt(zbr * zbr * zbr * zbr * zbr * bmagcgs) + 0.287 * zbr * zbr);
         double ia_by_b_num = bmagcgs / zeta_a_denom_num;
         kapparadnumsyn = c->kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * ia_by_b_num) * rho;
+        if (bridge) kapparadnumsyn *= (te / tc_n) / (1.0 + te / tc_n);
       }
       double bmb = bmagcgs > 0.0 ? kzm_pow(bmagcgs, -0.463) : 0.0;
       double ia_by_b_ross_rad = 0.13 * kzm_pow(bmagcgs, 0.69) /
@@ -517,15 +531,19 @@ void kzo_calc_opacities(const kz_opac_params *c, double rho, double tgas, double
         kappagassynross = c->kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * ia_by_b_ross_gas) * rho;
       }
     }
-    double terel = te * c->k_over_mecsq;
-    double terelfactor = (terel * terel) / (1.0 + terel * terel);
-    kappagassyn *= terelfactor;
-    if (want_rad) {
-      kapparadsyn *= terelfactor;
-      kapparadsynross *= terelfactor;
-      if (want_full) {
-        kapparadnumsyn *= terelfactor;
-        kappagassynross *= terelfactor;
+    if (bridge)
+      kappagassyn += c->kappacgs2gu * (2.35869e-21 * (nethcgs / rhocgs) * (bmagcgs * bmagcgs) / (te * te * te)) * rho;
+    if (!bridge) {
+      double terel = te * c->k_over_mecsq;
+      double terelfactor = (terel * terel) / (1.0 + terel * terel);
+      kappagassyn *= terelfactor;
+      if (want_rad) {
+        kapparadsyn *= terelfactor;
+        kapparadsynross *= terelfactor;
+        if (want_full) {
+          kapparadnumsyn *= terelfactor;
+          kappagassynross *= terelfactor;
+        }
       }
     }
   }

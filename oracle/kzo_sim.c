@@ -116,8 +116,7 @@ kzo_sim *kzo_sim_create(const kz_grid *gr, const kz_physics *ph, const double *g
                         const double *gconb1, const double *gb2, const double *gconb2, const double *bcx, int *err) {
   *err = 0;
   if (gr->ny <= 1 || gr->nx <= 1 || gr->ng < 3) { *err = KZ_E_UNSUPPORTED; return 0; }
-  if (ph->reduceorderatbh || ph->reduceorderafterfixup ||
-      ph->opac.synchrotron_bridge || ph->floors.b2rhofloorframe != 0) { *err = KZ_E_UNSUPPORTED; return 0; }
+  if (ph->floors.b2rhofloorframe < 0 || ph->floors.b2rhofloorframe > 1) { *err = KZ_E_INVALID_CONFIG; return 0; }
   if (ph->radviscosity && !ph->has_opac) { *err = KZ_E_INVALID_CONFIG; return 0; } /* sim.zig:335 */
   if (ph->correct_polaraxis && gr->ny <= 2 * ph->nccorrectpolar) { *err = KZ_E_INVALID_CONFIG; return 0; }
   kzo_sim *s = (kzo_sim *)calloc(1, sizeof(kzo_sim));
@@ -1227,10 +1226,19 @@ static int sweep_face(kzo_sim *s, int dim, long i, const long cellc[3], uint64_t
   }
   double dx5[5];
   for (int k = 0; k < 5; ++k) dx5[k] = csize(s, i - 2 + k, dim);
+  /* sim.zig:1316-1337: REDUCEORDERATBH (the cell's radial index inside the
+     horizon) and REDUCEORDERAFTERFIXUP (a fixup flag set on the cell by the
+     last u2p or implicit pass; ghost cells carry none) drop the order by one */
+  int reduce = s->ph.reduceorderatbh && cellc[0] <= s->ph.reduce_order_ix_max;
+  if (!reduce && s->ph.reduceorderafterfixup &&
+      (*flagp(s, KZ_FLAG_HD_FIXUP, cellc[0], cellc[1], cellc[2]) != 0 ||
+       *flagp(s, KZ_FLAG_RAD_FIXUP, cellc[0], cellc[1], cellc[2]) != 0 ||
+       *flagp(s, KZ_FLAG_RADIMP_FIXUP, cellc[0], cellc[1], cellc[2]) != 0))
+    reduce = 1;
   double pl[NV], pr[NV];
   for (int iv = 0; iv < NV; ++iv) {
     double u5[5] = {st[0][iv], st[1][iv], st[2][iv], st[3][iv], st[4][iv]};
-    kzo_reconstruct(2, s->ph.minmod_theta, u5, dx5, &pl[iv], &pr[iv]);
+    kzo_reconstruct(2 - reduce, s->ph.minmod_theta, u5, dx5, &pl[iv], &pr[iv]);
   }
   double ffl[NV], ffr[NV];
   if (dol) {

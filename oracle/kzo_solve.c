@@ -240,10 +240,15 @@ kz_u2p_result kzo_u2p_mhd(const double *uu_in, double *pp, const kzo_geo *gm, do
   return r;
 }
 
-/* invert.zig:474-715 checkFloorsMhd, drift-frame branch */
+/* invert.zig:474-715 checkFloorsMhd: the b^2/rho floor in the drift frame
+   (b2rhofloorframe 0) or the ZAMO frame (1) */
 int kzo_check_floors_mhd(double *pp, const kzo_geo *gm, double pgamma, const kz_floor_params *fp,
                          uint64_t *n_guard) {
   int ret = 0;
+  /* invert.zig:485-492: the ZAMO floor adds its delta to the conserveds of
+     the primitives as they enter, before any floor */
+  double uu_entry[NV] = {0};
+  if (fp->b2rhofloorframe == 1) kzo_p2u_mhd(pp, uu_entry, gm, pgamma);
   if (pp[RHO] < fp->rhofloor) {
     pp[RHO] = fp->rhofloor;
     ret = -1;
@@ -273,7 +278,36 @@ int kzo_check_floors_mhd(double *pp, const kzo_geo *gm, double pgamma, const kz_
       fuu = bsq / (fp->b2uuratiomax * pp[UU]);
       floored = 1;
     }
-    if (floored) {
+    if (floored && fp->b2rhofloorframe == 1) {
+      /* invert.zig:649-684 (ZAMOFRAME): the new mass drho = rho (f - 1) moves
+         with the normal observer n^mu = -alpha g^{mu 0}; its conserved vector
+         is added to the entry conserveds through B3 and the primitives are
+         recovered by the hot, then the entropy solver; if both fail the
+         fluid-frame backup scales rho and u by f */
+      double pporg[NV];
+      for (int iv = 0; iv < NV; ++iv) pporg[iv] = pp[iv];
+      double drho = pp[RHO] * (f - 1.0);
+      double etacon[4], etarel[4];
+      for (int i = 0; i < 4; ++i) etacon[i] = -gm->alpha * gm->G[i][0];
+      kzo_convert(etacon, VEL4, VELR, gm, 1, etarel);
+      double dpp[NV], duu[NV], uu[NV];
+      for (int iv = 0; iv < NV; ++iv) dpp[iv] = 0.0;
+      dpp[RHO] = drho;
+      dpp[VX] = etarel[1];
+      dpp[VY] = etarel[2];
+      dpp[VZ] = etarel[3];
+      kzo_p2u_mhd(dpp, duu, gm, pgamma);
+      for (int iv = 0; iv < NV; ++iv) uu[iv] = uu_entry[iv];
+      for (int iv = 0; iv <= B3; ++iv) uu[iv] += duu[iv];
+      int rt = kzo_u2p_solver_w(uu, pp, gm, 0, pgamma);
+      if (rt < 0) rt = kzo_u2p_solver_w(uu, pp, gm, 1, pgamma);
+      if (rt < 0) {
+        for (int iv = 0; iv < NV; ++iv) pp[iv] = pporg[iv];
+        pp[RHO] *= f;
+        pp[UU] *= f;
+      }
+      ret = -1;
+    } else if (floored) {
       double wold = pp[RHO] + pp[UU] * pgamma;
       pp[RHO] *= f;
       pp[UU] *= fp->isentropic_b2rhofloors ? f : fuu;

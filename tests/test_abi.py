@@ -44,14 +44,24 @@ def test_struct_sizes():
     assert C.sizeof(T.kz_grid) == 4 * 4 + 6 * 8
 
 
-@pytest.mark.parametrize("field", ["reduceorderatbh", "reduceorderafterfixup"])
-def test_create_rejects_unsupported(field):
+@pytest.mark.parametrize("frame", [-1, 2])
+def test_create_rejects_unknown_floor_frame(frame):
+    """koral/solve/invert.zig:35: the b^2/rho floor frame is drift (0) or
+    ZAMO (1)."""
     grid = T.grid_puffy(16, 12)
     tab = host.metric_tables(grid)
     phys = T.physics_puffy()
-    setattr(phys, field, 1)
-    with pytest.raises(RuntimeError, match="Unsupported|-7"):
+    phys.floors.b2rhofloorframe = frame
+    with pytest.raises(RuntimeError, match="InvalidConfig"):
         hip.Sim(grid, phys, tab)
+
+
+def test_create_rejects_collapsed_grid():
+    grid = T.grid_puffy(16, 12)
+    tab = host.metric_tables(grid)
+    grid.ny = 1
+    with pytest.raises(RuntimeError, match="Unsupported|-7"):
+        hip.Sim(grid, T.physics_puffy(), tab)
 
 
 def test_create_rejects_radvisc_without_opacities():
