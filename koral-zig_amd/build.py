@@ -10,12 +10,13 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
 LIB = HERE / "lib"
-HIP_SOURCES = [HERE / "csrc" / "kz_step.hip", HERE / "csrc" / "kz_cellops.hip"]
+HIP_SOURCES = [HERE / "csrc" / "kz_step.hip", HERE / "csrc" / "kz_options.hip", HERE / "csrc" / "kz_cellops.hip"]
 HOST_SOURCES = [HERE / "host" / "kz_host.cpp"]
 ARCH = "gfx950"
 
@@ -42,13 +43,37 @@ def _run(cmd):
         raise RuntimeError(f"build failed: {' '.join(map(str, cmd))}")
 
 
+def compile_hip(out, defines=None):
+    """Compile every HIP source to an object of its own, in parallel, and
+    link them into the shared library `out`.  `defines` maps macro names to
+    values (the tuner's knobs)."""
+    out = Path(out)
+    dflags = [f"-D{k}={v}" for k, v in (defines or {}).items()]
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="kz_build_") as td:
+        objs, procs = [], []
+        for src in HIP_SOURCES:
+            obj = Path(td) / (src.stem + ".o")
+            objs.append(obj)
+            cmd = [_hipcc(), *cflags, *dflags, "-c", "-o", str(obj), str(src)]
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        failed = []
+        for cmd, p in procs:
+            text, _ = p.communicate()
+            if p.returncode != 0:
+                sys.stderr.write(text)
+                failed.append(" ".join(map(str, cmd)))
+        if failed:
+            raise RuntimeError("build failed: " + "; ".join(failed))
+        tmp = out.with_name(out.name + ".tmp")
+        _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+        os.replace(tmp, out)
+    return out
+
+
 def build_hip():
     LIB.mkdir(exist_ok=True)
-    out = LIB / "libkz_hip.so"
-    tmp = LIB / "libkz_hip.so.tmp"
-    _run([_hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *map(str, HIP_SOURCES)])
-    os.replace(tmp, out)
-    return out
+    return compile_hip(LIB / "libkz_hip.so")
 
 
 def build_host():

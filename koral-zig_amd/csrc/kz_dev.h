@@ -367,16 +367,21 @@ KZD Opac calc_opacities(const kz_opac_params &c, double rho, double te, double t
     // opacities.zig:208-297: with the bridge the synchrotron Trad is clamped
     // up to TradBB^(4/3) / Te^(1/3) (TradBB = trad here: no photon-number
     // evolution), non-relativistic components are added to the gas and
-    // radiation absorption, and the Terelfactor suppression is dropped
+    // radiation absorption, and the Terelfactor suppression is dropped.  The
+    // reference adds the gas term after the Trad block; nothing there reads
+    // or writes kgsyn, so adding it here gives the same bits and ends the
+    // live range of its inputs before that block.
     const bool bridge = c.synchrotron_bridge != 0;
-    double trad_syn = trad;
-    if (bridge) {
-      const double trad_lim = kzm_pow(trad, 4.0 / 3.0) / kzm_pow(te, 1.0 / 3.0);
-      trad_syn = trad < trad_lim ? trad_lim : trad;
-    }
     double emis_syn = 3.61e-34 * (nethcgs / rhocgs) * te * te * bmagcgs * bmagcgs;
     kgsyn = c.kappacgs2gu * (emis_syn / bbenergy) * rho;
+    if (bridge)
+      kgsyn += c.kappacgs2gu * (2.35869e-21 * (nethcgs / rhocgs) * (bmagcgs * bmagcgs) / (te * te * te)) * rho;
     if (want_rad) {
+      double trad_syn = trad;
+      if (bridge) {
+        const double trad_lim = kzm_pow(trad, 4.0 / 3.0) / kzm_pow(te, 1.0 / 3.0);
+        trad_syn = trad < trad_lim ? trad_lim : trad;
+      }
       double nu_mb = 1.19e-13 * te * te;
       double zbr = c.k_boltz_cgs * trad_syn / c.h_cgs / nu_mb;
       double den = 1.79 * kzm_cbrt(zbr * zbr * zbr * zbr * zbr * bmagcgs * bmagcgs * bmagcgs * bmagcgs) +
@@ -412,9 +417,7 @@ KZD Opac calc_opacities(const kz_opac_params &c, double rho, double te, double t
       double iabr = 0.13 * pb69 / (pz169 * kzm_exp(1.6 * pz463 * bmb));
       krsynross = c.kappacgs2gu * (2.13e39 * (nethcgs / rhocgs) / te5 * iabr) * rho;
     }
-    if (bridge) {
-      kgsyn += c.kappacgs2gu * (2.35869e-21 * (nethcgs / rhocgs) * (bmagcgs * bmagcgs) / (te * te * te)) * rho;
-    } else {
+    if (!bridge) {
       double terel = te * c.k_over_mecsq;
       double tf = (terel * terel) / (1.0 + terel * terel);
       kgsyn *= tf;

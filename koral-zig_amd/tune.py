@@ -1,7 +1,7 @@
 """Occupancy autotuner for the step's heavy kernels.
 
 Each heavy kernel carries an `amdgpu_waves_per_eu` bound (KZ_WPE_* in
-csrc/kz_step.hip): a larger bound caps the registers per lane so that more
+csrc/kz_sim_dev.h): a larger bound caps the registers per lane so that more
 waves share a SIMD and hide each other's latency, at the price of spills.
 The best bound depends on the kernel, the grid and the compiler, so it is
 measured; so are the x width of the face flux kernel's y/z face tiles
@@ -21,7 +21,7 @@ margin picks noise).
 from the repository root, on a machine with the GPU.  The two halves can
 also run apart: `--build-only` compiles the libraries into --out (on any
 machine with hipcc), `--prebuilt` measures the libraries found there.  The
-defaults in kz_step.hip are the values this tuner selected.
+defaults in kz_sim_dev.h are the values this tuner selected.
 """
 import argparse
 import concurrent.futures as cf
@@ -73,12 +73,7 @@ def build_variant(name, defs, outdir):
     import build as B
     Path(outdir).mkdir(parents=True, exist_ok=True)
     lib = Path(outdir) / ("libkz_hip_" + name.replace("=", "_") + ".so")
-    cmd = [B._hipcc(), *B.HIPCC_FLAGS, *[f"-D{k}={v}" for k, v in defs.items()], "-o", str(lib),
-           *map(str, B.HIP_SOURCES)]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"build of {name} failed:\n{r.stderr[-2000:]}")
-    return lib
+    return B.compile_hip(lib, defs)
 
 
 def measure(lib, nx, ny, steps):
