@@ -67,7 +67,7 @@ BATCH_FLOPS_PER_CELL = 134811.4
 # FETCH_SIZE and WRITE_SIZE on this benchmark, each scaled by what the same
 # counter reads for a copy kernel of known size in the same run (k_soa2aos:
 # fetch x1.132, write x1.000).  Only valid for the default 384x360 workload.
-TRAFFIC_BYTES_PER_LAUNCH_C1 = {"implicit": 2.09e8, "update": 1.33e8, "face_flux": 1.27e8, "u2p": 5.77e7}
+TRAFFIC_BYTES_PER_LAUNCH_C1 = {"implicit": 1.68e8, "update": 1.39e8, "face_flux": 1.33e8, "u2p": 5.78e7}
 
 
 def _pkg():
