@@ -8,7 +8,9 @@
   whole steps with strong-field cells planted;
 * the synchrotron bridge of the opacities (koral/physics/opacities.zig:
   208-297), pinned in tests/test_oracle_synchrotron_bridge.py: the
-  opacity/radiation-state kernel, the implicit batch and whole steps.
+  opacity/radiation-state kernel, the implicit batch and whole steps;
+* all three in the members of a run group, whose implicit cells are solved
+  by cross-run launches.
 
 Each must give the oracle's state, flags, return codes and clock bit for
 bit.
@@ -226,3 +228,37 @@ def test_bridge_steps(dims):
         gpu.step()
         assert_same(cpu, gpu, f"step {k + 1}")
     gpu.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_group_with_options(mode):
+    """A run group whose members run the options: the bridge variants of the
+    cross-run implicit kernels (both launch shapes), and on each member's
+    stream the ZAMO-frame u2p and face kernels with the per-cell order choice;
+    strong-field cells planted in each member."""
+    pairs = []
+    for nx, ny, spin, seed, cells in ((24, 20, 0.0, 61, [(10, 8), (15, 12)]),
+                                      (32, 24, 0.5, 62, [(20, 10), (5, 6)])):
+        grid = T.grid_puffy(nx, ny, rmin=T.rmin_for_spin(spin))
+        ph = _copy(T.physics_puffy(bh_spin=spin), reduceorderafterfixup=1)
+        ph.floors.b2rhofloorframe = 1
+        ph.opac.synchrotron_bridge = 1
+        cpu, gpu = _pair(grid, ph, seed)
+        p0 = _plant_strong_field(cpu.get_state()[0], grid, cells)
+        for s in (cpu, gpu):
+            s.set_state(p=p0)
+            s.p2u_domain()
+        pairs.append((cpu, gpu))
+    print(f"group with options, implicit launch mode {mode}", flush=True)
+    grp = hip.Group([g for _, g in pairs])
+    grp.set_implicit_launch(mode)
+    grp.step_many(2)
+    grp.sync()
+    for cpu, _ in pairs:
+        cpu.step()
+        cpu.step()
+    for i, (cpu, gpu) in enumerate(pairs):
+        assert_same(cpu, gpu, f"run {i}")
+    grp.close()
+    for _, g in pairs:
+        g.close()

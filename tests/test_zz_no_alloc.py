@@ -72,6 +72,22 @@ def test_noalloc_matches_batch_twice():
         assert c_a == c_b and c_a[2] > 0.9 * n
 
 
+def test_noalloc_bridge_matches_batch():
+    """The synchrotron bridge on: the no-alloc entry point launches the
+    bridge variants of the implicit kernels, as the batch does."""
+    phys = T.physics_puffy()
+    phys.opac.synchrotron_bridge = 1
+    print("no-alloc solve of 8000 cells with the synchrotron bridge", flush=True)
+    pp, geom, dt = _inputs(8000, 13, phys)
+    uu_a, pp_a, r_a, c_a = _reference(pp, geom, dt, phys)
+    uu_b, pp_b, r_b, c_b = _noalloc(pp, geom, dt, phys)
+    for k in ("ok", "rung", "iters", "whichprim", "whicheq", "whichframe"):
+        assert np.array_equal(r_a[k], r_b[k]), k
+    assert pp_a.tobytes() == pp_b.tobytes()
+    assert uu_a.tobytes() == uu_b.tobytes()
+    assert c_a == c_b and c_a[2] > 0.9 * 8000
+
+
 def test_noalloc_rejects_oversize():
     phys = T.physics_puffy()
     rc = hip.lib().kz_implicit_solve_noalloc(MAX_CELLS + 1, None, None, None, None, None, None, None,
