@@ -216,13 +216,17 @@ def test_bridge_implicit_batch():
     assert (nfail, iters, solves) == (int((~ok).sum()), int(rc["iters"][ok].sum()), int(ok.sum()))
 
 
+@pytest.mark.parametrize("launch", [0, 2])
 @pytest.mark.parametrize("dims", ["2d", "3d"])
-def test_bridge_steps(dims):
+def test_bridge_steps(dims, launch):
+    """Launch 0 takes the shape the size selects (the merged first rung at
+    these sizes), 2 forces the persistent launches."""
     grid, seed = (T.grid_puffy(24, 20), 3) if dims == "2d" else (T.grid_puffy(16, 12, 6), 4)
     ph = T.physics_puffy()
     ph.opac.synchrotron_bridge = 1
     cpu, gpu = _pair(grid, ph, seed)
-    print(f"synchrotron bridge {dims}", flush=True)
+    gpu.set_implicit_launch(launch)
+    print(f"synchrotron bridge {dims}, implicit launch {launch}", flush=True)
     for k in range(3):
         cpu.step()
         gpu.step()
@@ -262,3 +266,39 @@ def test_group_with_options(mode):
     grp.close()
     for _, g in pairs:
         g.close()
+
+
+@pytest.mark.timeout(300)
+def test_c1_384x360_all_options_production_physics():
+    """The headline grid with every option on and the production physics
+    (radiative viscosity and the dynamo): the viscous face kernel with the
+    per-cell order and the ZAMO floor, the ZAMO u2p, and the bridge variants
+    of the wavespeeds and of the implicit kernels in the launch shape this
+    size selects; strong-field cells planted.  Two CFL steps, bit for bit."""
+    from test_gpu_fullsize import SEED, _threads
+    _threads()
+    grid = T.grid_puffy(384, 360)
+    mp = dict(host.MP_PUFFY)
+    ph = _copy(T.physics_puffy(radviscosity=True, dynamo=True), reduceorderatbh=1, reduceorderafterfixup=1,
+               reduce_order_ix_max=host.reduce_order_ix_max(grid, mp))
+    ph.floors.b2rhofloorframe = 1
+    ph.opac.synchrotron_bridge = 1
+    tab = host.metric_tables(grid)
+    bl = host.bl_tables(grid)
+    p0 = host.puffy_prepinit(grid, tab, ph, host.PUFFY_INIT)
+    cpu = kzo.Sim(grid, ph, tab)
+    gpu = hip.Sim(grid, ph, tab)
+    for s in (cpu, gpu):
+        s.set_bl_tables(bl)
+        host.init_puffy(s, grid, tab, ph, seed=SEED, p0=p0)
+    p1 = _plant_strong_field(cpu.get_state()[0], grid, [(40, 60), (200, 180), (300, 20)])
+    for s in (cpu, gpu):
+        s.set_state(p=p1)
+        s.p2u_domain()
+    print(f"384x360 with every option, ix_max {ph.reduce_order_ix_max}", flush=True)
+    for k in range(2):
+        cpu.step()
+        gpu.step()
+        assert_same(cpu, gpu, f"step {k + 1}")
+        print(f"384x360 options step {k + 1} identical", flush=True)
+    gpu.close()
